@@ -1,0 +1,559 @@
+// sweep.hip -- the fused finite-volume stage kernel for CDNA4 (gfx950).
+//
+// One wavefront owns a tile: a strip of HYPE_TILE_W (62) owned columns plus
+// one ring column on each side, and `hrows` owned rows.  Lane l works on
+// column x0-1+l and the wave marches up the strip one row per iteration:
+//
+//   * a rolling window of 6 rows x 68 columns of the stage state lives in
+//     LDS (component-major, so neighbouring lanes read neighbouring doubles).
+//     Each iteration loads one new row, synthesises ghost cells from the
+//     boundary index maps (the device replacement for DMGlobalToLocal and the
+//     boundary loops, rhs_function_prim.c:44-315) and converts it to
+//     primitives (ComputePrimitiveVariables, post_process.c:165-200);
+//   * every lane reconstructs its cell (Reconstruct, admissibility check and
+//     minmod fallback, rhs_function_prim.c:321-456) into the values at the
+//     face quadrature points;
+//   * an x-face pairs a lane with its left neighbour through a lane shuffle
+//     and a y-face pairs a row with the previous row kept in registers, so
+//     each Rusanov flux + non-conservative jump (riemann.c:112-140) is
+//     computed once per face of the tile with no face scratch in memory;
+//   * assembly (rhs_function_prim.c:589-602) and the SSP-RK stage update are
+//     fused; the stage output goes straight to HBM, and the wave's largest
+//     wave speed feeds the device-wide CFL maximum (:617-623).
+//
+// Arithmetic follows the reference's expression trees and is compiled
+// without FMA contraction, so results equal the CPU restatement bit for bit.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../../include/hype2d.h"
+#include "../hype_internal.h"
+#include "physics.hpp"
+#include "recon.hpp"
+
+namespace hype {
+
+// --------------------------------------------------------------- schemes --
+// Stage table of the SSP scheme PETSc uses for order N (hype.c:362-382).
+// Buffers: 0 = U (solution), 1 = A, 2 = B.
+// op 0 (AXPY):      out = in + (dt/bden) * L           (VecAXPY / VecAYPX)
+// op 1 (AXPBYPCZ):  out = (a*X + (dt/bden)*L) + g*Z     (VecAXPBYPCZ)
+struct StageDesc {
+  int in, out, xb, zb, op;
+  double a, g, bden;
+};
+
+__device__ __forceinline__ StageDesc stage_desc(int order, int s) {
+  if (order == 0)  // TSEULER: update = U + dt*F, copied back to U afterwards
+    return StageDesc{0, 1, 0, 0, 0, 0.0, 0.0, 1.0};
+  if (order == 1) {  // TSSSPRKS2, s = 2
+    if (s == 0) return StageDesc{0, 1, 0, 0, 0, 0.0, 0.0, 2 - 1.};
+    return StageDesc{1, 0, 1, 0, 1, (2 - 1.) / 2, 1. / 2, 2.0};
+  }
+  // TSSSPRKS3, s = 4 (n = 2, r = 2)
+  if (s == 0) return StageDesc{0, 1, 0, 0, 0, 0.0, 0.0, 2.0};
+  if (s == 1) return StageDesc{1, 2, 0, 0, 0, 0.0, 0.0, 2.0};
+  if (s == 2) return StageDesc{2, 1, 0, 2, 1, 1. * 2 / (2 * 2 - 1), (2 - 1.) / (2 * 2 - 1), 2 * (2 * 2 - 1)};
+  return StageDesc{1, 0, 0, 0, 0, 0.0, 0.0, 2.0};
+}
+
+__device__ __forceinline__ int num_stages_dev(int order) { return order == 0 ? 1 : (order == 1 ? 2 : 4); }
+
+// packed error word (layout in hype_internal.h)
+__device__ __forceinline__ unsigned long long pack_err(int seq, long key, int kind, double val) {
+  const float f = (float)val;
+  return ((unsigned long long)(seq & 0x7) << HYPE_ERR_SEQ_SHIFT) |
+         ((unsigned long long)(key & 0xFFFFFFF) << HYPE_ERR_FACE_SHIFT) |
+         ((unsigned long long)(kind & 0xF) << HYPE_ERR_KIND_SHIFT) | (unsigned long long)(__float_as_uint(f) >> 3);
+}
+
+// map a (possibly ghost) index onto the owned range; toggles flip for a
+// reflective side (rhs_function_prim.c:84-99,152-167 / 213-228,266-281)
+__device__ __forceinline__ int bc_map(int i, int n, int bclo, int bchi, bool& flip) {
+  if (i < 0) {
+    if (bclo == HYPE_BC_TRANSMISSIVE) return 0;
+    if (bclo == HYPE_BC_REFLECTIVE) {
+      flip = true;
+      return -1 - i;
+    }
+    return i + n;  // periodic wrap (DMDA_BOUNDARY_PERIODIC)
+  }
+  if (i >= n) {
+    if (bchi == HYPE_BC_TRANSMISSIVE) return n - 1;
+    if (bchi == HYPE_BC_REFLECTIVE) {
+      flip = true;
+      return 2 * (n - 1) - i + 1;
+    }
+    return i - n;
+  }
+  return i;
+}
+
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) {
+    double w = __shfl_xor(v, o, HYPE_WAVE);
+    v = (v < w) ? w : v;
+  }
+  return v;
+}
+
+__device__ __forceinline__ unsigned long long wave_min_u64(unsigned long long v) {
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) {
+    unsigned long long w = __shfl_xor(v, o, HYPE_WAVE);
+    v = (w < v) ? w : v;
+  }
+  return v;
+}
+
+constexpr int NSLOT = 6;
+constexpr int COLS = HYPE_WAVE + 4;
+
+template <class S, bool PRIM, int N>
+struct Tile {
+  static constexpr int NV = S::NV;
+  using Qd = Quad<N>;
+  static constexpr int NDOF = Qd::NDOF, NGF = Qd::NGF, NGV = Qd::NGV, NNODE = Qd::NNODE;
+
+  // values at the face points and the volume points of one cell from its
+  // modal coefficients (rhs_function_prim.c:382-388, 425-456)
+  __device__ static __forceinline__ void eval_points(const double* co, int c, double (&fv)[NNODE][NV],
+                                                     double (&vv)[NGV][NV], double (&vgx)[NGV][NV],
+                                                     double (&vgy)[NGV][NV], double r1_h) {
+#pragma unroll
+    for (int p = 0; p < NNODE; ++p) {
+#define PHIN(k) Qd::phin(p, k)
+      HYPE_BASIS_SUM(fv[p][c], co, NDOF, PHIN);
+#undef PHIN
+    }
+    if constexpr (S::HAS_NCP) {
+#pragma unroll
+      for (int q = 0; q < NGV; ++q) {
+        double gx, gy;
+#define PHIV(k) Qd::phiv(q, k)
+#define GPX(k) Qd::gphx(q, k)
+#define GPY(k) Qd::gphy(q, k)
+        HYPE_BASIS_SUM(vv[q][c], co, NDOF, PHIV);
+        HYPE_BASIS_SUM(gx, co, NDOF, GPX);
+        HYPE_BASIS_SUM(gy, co, NDOF, GPY);
+#undef PHIV
+#undef GPX
+#undef GPY
+        vgx[q][c] = r1_h * gx;
+        vgy[q][c] = r1_h * gy;
+      }
+    }
+  }
+
+  __device__ static void run(const hype_launch& L, int tile_id, double* win, int lane) {
+    const hype_tile T = L.tiles[tile_id];
+    const hype_inst* I = &L.insts[T.inst];
+    hype_dstate* st = I->st;
+    const int nx = I->nx, ny = I->ny;
+    const int bcL = I->bc[0], bcR = I->bc[1], bcB = I->bc[2], bcT = I->bc[3];
+    const double r1_h = I->r1_h;
+    const Eos eos{I->g1, I->g2, I->p1, I->p2, (I->quirks & HYPE_QUIRK_BN_P2_IS_PHI) != 0};
+    const bool leader = (T.flags & 1) != 0;
+    const int mode = L.mode;
+
+    // ---- time state (single writer per field per launch) -------------
+    double dt = 0.0;
+    int cur = 0;
+    StageDesc sd = stage_desc(N, L.stage);
+    const bool last_stage = (mode == HYPE_MODE_STAGE) && (L.stage == num_stages_dev(N) - 1);
+    if (mode == HYPE_MODE_STAGE) {
+      if (L.stage == 0) {
+        const int step = st->step;
+        const double t = st->t;
+        // stop after max_steps, at t_end, or after a step (or the priming
+        // call) that met an inadmissible state
+        if (step >= st->max_steps || !(t < st->t_end) || st->err[(step + 1) & 1] != ~0ull) {
+          if (leader && lane == 0) st->done = 1;
+          return;
+        }
+        const double sm = __longlong_as_double((long long)st->smax[step & 1]);
+        dt = (I->cfl * I->h) / (2.0 * sm);
+        if (st->clip && t + dt > st->t_end) dt = st->t_end - t;
+        cur = step;
+        if (leader && lane == 0) {
+          st->dt = dt;
+          st->cur = step;
+          st->smax[(step + 1) & 1] = 0ull;
+          if (st->dt_hist && step < st->hist_len) st->dt_hist[step] = dt;
+        }
+      } else {
+        if (st->done) return;
+        dt = st->dt;
+        cur = st->cur;
+      }
+      if (last_stage && leader && lane == 0) {
+        st->t = st->t + dt;
+        st->step = cur + 1;
+        st->err[(cur + 1) & 1] = ~0ull;  // the next step records there
+      }
+    }
+    const double* Uin = (mode == HYPE_MODE_STAGE) ? I->buf[sd.in] : (L.rhs_in ? L.rhs_in : I->buf[0]);
+    const double cdt = dt / sd.bden;
+
+    // ---- geometry ------------------------------------------------------
+    const int x0 = T.x0, y0 = T.y0;
+    const int W = min(HYPE_TILE_W, nx - x0);
+    const int H = min(I->hrows, ny - y0);
+    const int col = x0 - 1 + lane;
+    const bool owned = lane >= 1 && lane <= W;
+    const bool xlane = lane >= 1 && lane <= W + 1;
+    const int cw = lane + 2;                 // window column of this lane's cell
+    const int rbase = y0 - 3;                // window row of slot 0
+    auto slot = [&](int r) { return (r - rbase) % NSLOT; };
+    auto W_ = [&](int s, int c, int k) -> double& { return win[(s * NV + c) * COLS + k]; };
+
+    // ---- row loader ----------------------------------------------------
+    auto fetch = [&](int rr, int k, double* q) {
+      const int cc = min(x0 - 3 + k, nx + 2);
+      bool fx = false, fy = false;
+      const int sc = bc_map(cc, nx, bcL, bcR, fx);
+      const int sr = bc_map(rr, ny, bcB, bcT, fy);
+      const double* p = Uin + ((size_t)sr * nx + sc) * NV;
+#pragma unroll
+      for (int c = 0; c < NV; ++c) q[c] = p[c];
+      if (fx) {
+        q[S::REFX[0]] = -q[S::REFX[0]];
+        if (S::REFX[1] >= 0) q[S::REFX[1]] = -q[S::REFX[1]];
+      }
+      if (fy) {
+        q[S::REFY[0]] = -q[S::REFY[0]];
+        if (S::REFY[1] >= 0) q[S::REFY[1]] = -q[S::REFY[1]];
+      }
+    };
+    auto put = [&](int s, int k, const double* q) {
+      double v[NV];
+      if (PRIM) {
+        S::cons2prim(eos, q, v);
+      } else {
+#pragma unroll
+        for (int c = 0; c < NV; ++c) v[c] = q[c];
+      }
+#pragma unroll
+      for (int c = 0; c < NV; ++c) W_(s, c, k) = v[c];
+    };
+
+    // prologue: rows y0-3 .. y0+1
+    for (int rr = y0 - 3; rr <= y0 + 1; ++rr) {
+      double qa[NV], qb[NV];
+      fetch(rr, lane, qa);
+      if (lane < 4) fetch(rr, lane + HYPE_WAVE, qb);
+      put(slot(rr), lane, qa);
+      if (lane < 4) put(slot(rr), lane + HYPE_WAVE, qb);
+    }
+    __syncthreads();
+
+    // ---- carried state -------------------------------------------------
+    double topPrev[NGF][NV];
+    double Gprev[NV], Eprev[NV], xpartPrev[NV], ncpPrev[NV];
+#pragma unroll
+    for (int c = 0; c < NV; ++c) {
+      Gprev[c] = 0.0;
+      Eprev[c] = 0.0;
+      xpartPrev[c] = 0.0;
+      ncpPrev[c] = 0.0;
+#pragma unroll
+      for (int q = 0; q < NGF; ++q) topPrev[q][c] = 0.0;
+    }
+    double smax = 0.0;
+    unsigned long long err = ~0ull;
+    const int seq = (mode == HYPE_MODE_STAGE) ? L.stage + 1 : 0;
+    const double WF = Qd::WF, WV = Qd::WV;
+
+    for (int r = y0 - 1; r <= y0 + H; ++r) {
+      // prefetch the row that enters the window after this iteration
+      const bool more = (r + 3 <= y0 + H + 2);
+      double qa[NV], qb[NV];
+      if (more) {
+        fetch(r + 3, lane, qa);
+        if (lane < 4) fetch(r + 3, lane + HYPE_WAVE, qb);
+      }
+
+      const bool ownedRow = (r >= y0) && (r < y0 + H);
+      const bool doY = (r >= y0);
+      const bool doAsm = (r >= y0 + 1);
+      const int sm2 = slot(r - 2), sm1 = slot(r - 1), s0 = slot(r), sp1 = slot(r + 1), sp2 = slot(r + 2);
+
+      // ---- reconstruction ----------------------------------------------
+      double fv[NNODE][NV], vv[NGV][NV], vgx[NGV][NV], vgy[NGV][NV];
+#pragma unroll
+      for (int c = 0; c < NV; ++c) {
+        double ux[5], uy[5], ud[5], co[NDOF];
+        ux[0] = W_(s0, c, cw - 2);
+        ux[1] = W_(s0, c, cw - 1);
+        ux[2] = W_(s0, c, cw);
+        ux[3] = W_(s0, c, cw + 1);
+        ux[4] = W_(s0, c, cw + 2);
+        uy[0] = W_(sm2, c, cw);
+        uy[1] = W_(sm1, c, cw);
+        uy[2] = ux[2];
+        uy[3] = W_(sp1, c, cw);
+        uy[4] = W_(sp2, c, cw);
+        ud[0] = ux[2];
+        ud[1] = W_(sp1, c, cw + 1);
+        ud[2] = W_(sm1, c, cw + 1);
+        ud[3] = W_(sp1, c, cw - 1);
+        ud[4] = W_(sm1, c, cw - 1);
+        reconstruct<N>(ux, uy, ud, co);
+        eval_points(co, c, fv, vv, vgx, vgy, r1_h);
+      }
+      bool pad = true;
+#pragma unroll
+      for (int p = 0; p < NNODE; ++p) pad = pad && (PRIM ? S::pad_prim(eos, fv[p]) : S::pad(eos, fv[p]));
+      if (!pad) {  // rhs_function_prim.c:407-421 / rhs_function.c:409-421
+#pragma unroll
+        for (int c = 0; c < NV; ++c) {
+          double co[NDOF];
+          const double w0 = W_(s0, c, cw);
+          co[0] = w0;
+          if constexpr (NDOF > 1) {
+            co[1] = PRIM ? minmod(W_(s0, c, cw + 1) - w0, w0 - W_(s0, c, cw - 1)) : 0.0;
+            co[2] = PRIM ? minmod(W_(sp1, c, cw) - w0, w0 - W_(sm1, c, cw)) : 0.0;
+          }
+#pragma unroll
+          for (int k = 3; k < NDOF; ++k) co[k] = 0.0;
+          eval_points(co, c, fv, vv, vgx, vgy, r1_h);
+        }
+      }
+      if (L.fallback && ownedRow && owned) L.fallback[(size_t)r * nx + col] = (unsigned char)(!pad);
+
+      // ---- volume non-conservative product (rhs_function_prim.c:461-479)
+      double ncp[NV];
+#pragma unroll
+      for (int c = 0; c < NV; ++c) ncp[c] = 0.0;
+      if constexpr (S::HAS_NCP) {
+        if (mode != HYPE_MODE_PRIME && ownedRow) {
+#pragma unroll
+          for (int q = 0; q < NGV; ++q) {
+            double B[NV];
+            S::template vol_ncp<PRIM>(eos, vv[q], vgx[q], vgy[q], B);
+#pragma unroll
+            for (int c = 0; c < NV; ++c)
+              if (S::DNZ[c]) ncp[c] = ncp[c] + -WV * B[c];
+          }
+        }
+      }
+
+      // ---- x-faces i-1/2 (rhs_function_prim.c:485-533) -------------------
+      double xpart[NV];
+      {
+        double Fx[NV], Dx[NV];
+#pragma unroll
+        for (int c = 0; c < NV; ++c) {
+          Fx[c] = 0.0;
+          Dx[c] = 0.0;
+        }
+#pragma unroll
+        for (int q = 0; q < NGF; ++q) {
+          double vl[NV], vr[NV], Fq[NV], Dq[NV];
+#pragma unroll
+          for (int c = 0; c < NV; ++c) {
+            vl[c] = __shfl_up(fv[Qd::R(q)][c], 1, HYPE_WAVE);
+            vr[c] = fv[Qd::L(q)][c];
+          }
+          if (ownedRow) {
+            Bad b{0, 0.0};
+            const double s = riemann<S, PRIM, 0>(eos, vl, vr, Fq, Dq, b);
+            if (xlane) {
+              if (s > smax) smax = s;
+              if (b.kind) {
+                const long key = ((long)r * (nx + 1) + col) * NGF + q;
+                const unsigned long long pk = pack_err(seq, key, b.kind, b.val);
+                if (pk < err) err = pk;
+              }
+            }
+#pragma unroll
+            for (int c = 0; c < NV; ++c) {
+              Fx[c] = Fx[c] + WF * Fq[c];
+              if (S::DNZ[c]) Dx[c] = Dx[c] + WF * Dq[c];
+            }
+          }
+        }
+#pragma unroll
+        for (int c = 0; c < NV; ++c) {
+          const double Fr = __shfl_down(Fx[c], 1, HYPE_WAVE);
+          const double Dr = S::DNZ[c] ? __shfl_down(Dx[c], 1, HYPE_WAVE) : 0.0;
+          xpart[c] = -r1_h * (Fr - Fx[c] + 0.5 * (Dr + Dx[c]));
+        }
+      }
+
+      // ---- y-face r-1/2 (rhs_function_prim.c:537-585) --------------------
+      double Gy[NV], Ey[NV];
+#pragma unroll
+      for (int c = 0; c < NV; ++c) {
+        Gy[c] = 0.0;
+        Ey[c] = 0.0;
+      }
+      if (doY) {
+#pragma unroll
+        for (int q = 0; q < NGF; ++q) {
+          double Fq[NV], Dq[NV];
+          Bad b{0, 0.0};
+          const double s = riemann<S, PRIM, 1>(eos, topPrev[q], fv[Qd::B(q)], Fq, Dq, b);
+          if (owned) {
+            if (s > smax) smax = s;
+            if (b.kind) {
+              const long key = (long)ny * (nx + 1) * NGF + ((long)r * nx + col) * NGF + q;
+              const unsigned long long pk = pack_err(seq, key, b.kind, b.val);
+              if (pk < err) err = pk;
+            }
+          }
+#pragma unroll
+          for (int c = 0; c < NV; ++c) {
+            Gy[c] = Gy[c] + WF * Fq[c];
+            if (S::DNZ[c]) Ey[c] = Ey[c] + WF * Dq[c];
+          }
+        }
+      }
+
+      // ---- assembly + stage update of cell (col, r-1) --------------------
+      if (doAsm && owned && mode != HYPE_MODE_PRIME) {
+        const size_t idx = ((size_t)(r - 1) * nx + col) * NV;
+        double rhs[NV];
+#pragma unroll
+        for (int c = 0; c < NV; ++c) {
+          const double Y = Gy[c] - Gprev[c] + 0.5 * (Ey[c] + Eprev[c]);
+          const double t = xpartPrev[c] - r1_h * Y;
+          rhs[c] = ((S::HAS_NCP && S::DNZ[c]) ? ncpPrev[c] : 0.0) + t;
+        }
+        if (mode == HYPE_MODE_RHS) {
+          double* out = L.rhs_out ? L.rhs_out : I->buf[1];
+#pragma unroll
+          for (int c = 0; c < NV; ++c) out[idx + c] = rhs[c];
+        } else {
+          double* out = I->buf[sd.out];
+          if (sd.op == 0) {
+#pragma unroll
+            for (int c = 0; c < NV; ++c) out[idx + c] = Uin[idx + c] + cdt * rhs[c];
+          } else {
+            const double* X = I->buf[sd.xb];
+            const double* Z = I->buf[sd.zb];
+#pragma unroll
+            for (int c = 0; c < NV; ++c) out[idx + c] = sd.a * X[idx + c] + cdt * rhs[c] + sd.g * Z[idx + c];
+          }
+        }
+      }
+
+      // ---- carry -----------------------------------------------------------
+#pragma unroll
+      for (int c = 0; c < NV; ++c) {
+        Gprev[c] = Gy[c];
+        Eprev[c] = Ey[c];
+        xpartPrev[c] = xpart[c];
+        ncpPrev[c] = ncp[c];
+#pragma unroll
+        for (int q = 0; q < NGF; ++q) topPrev[q][c] = fv[Qd::T(q)][c];
+      }
+
+      // ---- slide the window ------------------------------------------------
+      if (more) {
+        const int sn = slot(r + 3);
+        put(sn, lane, qa);
+        if (lane < 4) put(sn, lane + HYPE_WAVE, qb);
+      }
+      __syncthreads();
+    }
+
+    // ---- CFL maximum and error word ----------------------------------------
+    const bool want_smax = (mode != HYPE_MODE_STAGE) || last_stage;
+    if (want_smax) {
+      const double m = wave_max(smax);
+      if (lane == 0) {
+        unsigned long long* slotp =
+            (mode == HYPE_MODE_RHS) ? &st->smax_rhs : (mode == HYPE_MODE_PRIME ? &st->smax[st->step & 1] : &st->smax[(cur + 1) & 1]);
+        atomicMax(slotp, (unsigned long long)__double_as_longlong(m));
+      }
+    }
+    const unsigned long long e = wave_min_u64(err);
+    if (lane == 0 && e != ~0ull) {
+      unsigned long long* ep =
+          (mode == HYPE_MODE_RHS) ? &st->err_rhs : (mode == HYPE_MODE_PRIME ? &st->err[1] : &st->err[cur & 1]);
+      atomicMin(ep, e);
+    }
+  }
+};
+
+template <class S, bool PRIM, int N>
+__global__ __launch_bounds__(HYPE_WAVE) void stage_kernel(hype_launch L) {
+  __shared__ double win[NSLOT * S::NV * COLS];
+  const int lane = threadIdx.x;
+  if (L.grid <= 0) {
+    if ((int)blockIdx.x < L.ntiles) Tile<S, PRIM, N>::run(L, blockIdx.x, win, lane);
+    return;
+  }
+  for (;;) {
+    unsigned t = 0;
+    if (lane == 0) t = atomicAdd(&L.queue[0], 1u);
+    const int tile = __builtin_amdgcn_readfirstlane((int)__shfl(t, 0, HYPE_WAVE));
+    if (tile >= L.ntiles) break;
+    Tile<S, PRIM, N>::run(L, tile, win, lane);
+    __syncthreads();
+  }
+  if (lane == 0) {
+    __threadfence();
+    const unsigned fin = atomicAdd(&L.queue[1], 1u);
+    if (fin == gridDim.x - 1) {
+      atomicExch(&L.queue[0], 0u);
+      atomicExch(&L.queue[1], 0u);
+    }
+  }
+}
+
+template <class S, bool PRIM, int N>
+static hipError_t launch_variant(const hype_launch& L, hipStream_t s) {
+  const int grid = L.grid > 0 ? L.grid : L.ntiles;
+  hipLaunchKernelGGL((stage_kernel<S, PRIM, N>), dim3(grid), dim3(HYPE_WAVE), 0, s, L);
+  return hipGetLastError();
+}
+
+template <class S>
+static hipError_t launch_sys(const hype_launch& L, hipStream_t s) {
+  const bool p = L.v.prim != 0;
+  switch (L.v.order) {
+    case 0: return p ? launch_variant<S, true, 0>(L, s) : launch_variant<S, false, 0>(L, s);
+    case 1: return p ? launch_variant<S, true, 1>(L, s) : launch_variant<S, false, 1>(L, s);
+    default: return p ? launch_variant<S, true, 2>(L, s) : launch_variant<S, false, 2>(L, s);
+  }
+}
+
+hipError_t launch_stage(const hype_launch& L, hipStream_t s) {
+  if (L.ntiles <= 0) return hipSuccess;
+  switch (L.v.system) {
+    case HYPE_EULER: return launch_sys<Euler>(L, s);
+    case HYPE_KAPILA: return launch_sys<Kapila>(L, s);
+    default: return launch_sys<BN>(L, s);
+  }
+}
+
+// ---------------------------------------------------- primitive pass --------
+template <class S>
+__global__ void cons2prim_kernel(const double* U, double* W, long n, Eos e) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double q[S::NV], v[S::NV];
+#pragma unroll
+  for (int c = 0; c < S::NV; ++c) q[c] = U[i * S::NV + c];
+  S::cons2prim(e, q, v);
+#pragma unroll
+  for (int c = 0; c < S::NV; ++c) W[i * S::NV + c] = v[c];
+}
+
+hipError_t launch_cons2prim(int system, const double* U, double* W, long n, Eos e, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  const int bs = 256;
+  const unsigned grid = (unsigned)((n + bs - 1) / bs);
+  if (system == HYPE_EULER)
+    hipLaunchKernelGGL(cons2prim_kernel<Euler>, dim3(grid), dim3(bs), 0, s, U, W, n, e);
+  else if (system == HYPE_KAPILA)
+    hipLaunchKernelGGL(cons2prim_kernel<Kapila>, dim3(grid), dim3(bs), 0, s, U, W, n, e);
+  else
+    hipLaunchKernelGGL(cons2prim_kernel<BN>, dim3(grid), dim3(bs), 0, s, U, W, n, e);
+  return hipGetLastError();
+}
+
+}  // namespace hype

@@ -1,0 +1,136 @@
+/*
+ * hype_internal.h -- data shared by the C host layer and the HIP device layer.
+ *
+ * Plain C structs only.  The host layer (csrc/host) fills them; the device
+ * layer (csrc/hip) launches kernels over them through the thin C ABI
+ * declared at the bottom (hyped_ functions).
+ */
+#ifndef HYPE_INTERNAL_H
+#define HYPE_INTERNAL_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define HYPE_MAXVAR 9
+
+/* strip geometry of the line-sweep kernel: one wavefront per tile, lanes
+ * cover the ring column on each side plus HYPE_TILE_W owned columns */
+#define HYPE_WAVE 64
+#define HYPE_TILE_W (HYPE_WAVE - 2)
+
+/* device-resident time state of one solve */
+typedef struct hype_dstate {
+    double t;              /* time at the start of the current step */
+    double dt;             /* dt of the current step (written by stage 0) */
+    double t_end;
+    double pad0;
+    unsigned long long smax[2];   /* CFL max wave speed slots, double bits */
+    unsigned long long err[2];    /* packed first inadmissible face of the
+                                     step of that parity, ~0 = none; the
+                                     priming call records into err[1] */
+    unsigned long long smax_rhs;  /* slots of single RHS evaluations */
+    unsigned long long err_rhs;
+    int step;              /* steps completed */
+    int cur;               /* index of the step in flight (written by stage 0) */
+    int max_steps;
+    int done;
+    int clip;
+    int hist_len;
+    double *dt_hist;       /* optional: dt of every step */
+} hype_dstate;
+
+/* per-instance descriptor read by the kernels */
+typedef struct hype_inst {
+    double *buf[4];        /* U (solution), A, B, C (stage work vectors) */
+    hype_dstate *st;       /* device time state */
+    int nx, ny;
+    int bc[4];
+    int hrows;             /* rows per tile of this instance */
+    int ntiles_x;
+    double h, r1_h, cfl;
+    double x_min, y_min;
+    double g1, g2, p1, p2;
+    unsigned quirks;
+    int pad1;
+} hype_inst;
+
+/* tile: instance, first owned column, first owned row, flags (bit 0 =
+ * leader tile of its instance, updates the time state) */
+typedef struct hype_tile {
+    int inst, x0, y0, flags;
+} hype_tile;
+
+/* packed error word: call sequence within the step (3 bits: 0 priming or
+ * single RHS, 1.. stage), face key (28), check kind (4), the offending value
+ * as the top 29 bits of its float32 image.  The smallest word is the lowest
+ * face of the earliest failing call, as the reference aborts there. */
+#define HYPE_ERR_SEQ_SHIFT 61
+#define HYPE_ERR_FACE_SHIFT 33
+#define HYPE_ERR_KIND_SHIFT 29
+
+/* kernel modes */
+#define HYPE_MODE_STAGE 0
+#define HYPE_MODE_RHS 1
+#define HYPE_MODE_PRIME 2
+
+/* kernel variant: system x path x order */
+typedef struct hype_variant {
+    int system, prim, order;
+} hype_variant;
+
+typedef struct hype_launch {
+    hype_variant v;
+    const hype_inst *insts;    /* device array */
+    const hype_tile *tiles;    /* device array */
+    int ntiles;
+    int mode;
+    int stage;
+    int grid;                  /* persistent blocks; 0 = one per tile */
+    unsigned *queue;           /* device {next, finished} counters */
+    const double *rhs_in;      /* RHS/PRIME mode: input state override */
+    double *rhs_out;           /* RHS mode: output */
+    unsigned char *fallback;   /* RHS mode: optional per-cell fallback mask */
+} hype_launch;
+
+/* number of stages of the scheme for order N */
+int hyped_num_stages(int order);
+
+/* ---- thin device ABI (implemented in csrc/hip/device.hip) -------------- */
+int hyped_init(void);
+int hyped_malloc(void **p, size_t bytes);
+int hyped_free(void *p);
+int hyped_memcpy_h2d(void *dst, const void *src, size_t bytes, void *stream);
+int hyped_memcpy_d2h(void *dst, const void *src, size_t bytes, void *stream);
+int hyped_memcpy_d2d(void *dst, const void *src, size_t bytes, void *stream);
+int hyped_fill_u64(void *dst, unsigned long long value, size_t count, void *stream);
+int hyped_stream_create(void **s);
+int hyped_stream_destroy(void *s);
+int hyped_stream_sync(void *s);
+int hyped_launch(const hype_launch *L, void *stream);
+int hyped_cons2prim(int system, const double *U, double *W, long ncells,
+                    double g1, double g2, double p1, double p2, void *stream);
+int hyped_copy(double *dst, const double *src, long n, void *stream);
+int hyped_mem_info(size_t *free_bytes, size_t *total_bytes);
+int hyped_num_cus(void);
+const char *hyped_last_error(void);
+
+/* HIP events on a stream */
+int hyped_event_create(void **ev);
+int hyped_event_destroy(void *ev);
+int hyped_event_record(void *ev, void *stream);
+int hyped_event_elapsed_ms(void *a, void *b, float *ms);
+
+/* graph capture of a sequence of launches on one stream */
+int hyped_graph_begin(void *stream);
+int hyped_graph_end(void *stream, void **graph_exec);
+int hyped_graph_launch(void *graph_exec, void *stream);
+int hyped_graph_destroy(void *graph_exec);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,295 @@
+"""ctypes binding of libhype2d.so (the C ABI declared in include/hype2d.h).
+
+Mirrors the reference's callback interface: ``Context.rhs(U, t)`` is the
+RHSFunctionPrim / RHSFunction call (HyPE-2D/rhs_function_prim.c:12,
+rhs_function.c:12) on the DMDA AoS layout, returning the RHS and the dt side
+effect; ``Context.solve`` drives the device-resident SSP-RK loop of
+hype.c:337-384.  The binding refuses to run without the compiled library:
+there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libhype2d.so")
+
+OK, E_INVALID, E_UNSUPPORTED, E_DEVICE, E_NOMEM, E_INADMISSIBLE, E_BUSY, E_NOTFOUND = range(8)
+EULER, KAPILA, BAER_NUNZIATO = 0, 1, 2
+BC_WALL, BC_INLET, BC_PERIODIC, BC_REFLECTIVE, BC_TRANSMISSIVE = range(5)
+QUIRK_BN_P2_IS_PHI = 0x1
+MEM_HOST, MEM_DEVICE = 0, 1
+TS_CLIP, TS_GRAPH = 0x1, 0x2
+NVAR = {EULER: 4, KAPILA: 6, BAER_NUNZIATO: 9}
+
+
+class HypeError(RuntimeError):
+    def __init__(self, code, msg=""):
+        super().__init__(f"hype2d error {code}: {msg}")
+        self.code = code
+        self.msg = msg
+
+
+class Problem(C.Structure):
+    _fields_ = [
+        ("system", C.c_int),
+        ("order", C.c_int),
+        ("recons_prim", C.c_int),
+        ("nx", C.c_int),
+        ("ny", C.c_int),
+        ("x_min", C.c_double),
+        ("x_max", C.c_double),
+        ("y_min", C.c_double),
+        ("y_max", C.c_double),
+        ("cfl", C.c_double),
+        ("bc", C.c_int * 4),
+        ("gamma1", C.c_double),
+        ("gamma2", C.c_double),
+        ("pi1", C.c_double),
+        ("pi2", C.c_double),
+        ("quirks", C.c_uint),
+    ]
+
+
+class Status(C.Structure):
+    _fields_ = [("t", C.c_double), ("dt", C.c_double), ("steps", C.c_int), ("done", C.c_int), ("err", C.c_int)]
+
+
+class BatchOpts(C.Structure):
+    _fields_ = [("tile_rows", C.c_int), ("use_graph", C.c_int), ("persistent_blocks", C.c_int)]
+
+
+class ServerOpts(C.Structure):
+    _fields_ = [
+        ("hbm_budget_bytes", C.c_size_t),
+        ("max_active", C.c_int),
+        ("steps_per_round", C.c_int),
+        ("tile_rows", C.c_int),
+    ]
+
+
+class Request(C.Structure):
+    _fields_ = [
+        ("problem", Problem),
+        ("U0", C.POINTER(C.c_double)),
+        ("t_end", C.c_double),
+        ("max_steps", C.c_int),
+    ]
+
+
+def make_problem(system, nx, ny, *, order=2, prim=True, domain=(0.0, 1.0, 0.0, 1.0), cfl=0.5,
+                 bc=(BC_TRANSMISSIVE,) * 4, gamma1=1.4, gamma2=1.4, pi1=0.0, pi2=0.0, quirks=0):
+    p = Problem()
+    p.system, p.order, p.recons_prim = system, order, 1 if prim else 0
+    p.nx, p.ny = nx, ny
+    p.x_min, p.x_max, p.y_min, p.y_max = domain
+    p.cfl = cfl
+    for k in range(4):
+        p.bc[k] = bc[k]
+    p.gamma1, p.gamma2, p.pi1, p.pi2 = gamma1, gamma2, pi1, pi2
+    p.quirks = quirks
+    return p
+
+
+_lib = None
+
+
+def lib():
+    """Load libhype2d.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise HypeError(E_DEVICE, f"{LIB_PATH} is missing; build it with `make -C hype-parallel_amd`")
+    L = C.CDLL(LIB_PATH)
+    P = C.POINTER
+    dp = P(C.c_double)
+    vp = C.c_void_p
+    L.hype_nvar.argtypes = [C.c_int]
+    L.hype_ctx_create.argtypes = [P(Problem), P(vp)]
+    L.hype_ctx_destroy.argtypes = [vp]
+    L.hype_ctx_destroy.restype = None
+    L.hype_rhs.argtypes = [vp, C.c_double, dp, dp, dp, C.c_int]
+    L.hype_rhs_diag.argtypes = [vp, C.c_double, dp, dp, dp, P(C.c_ubyte), C.c_int]
+    L.hype_cons2prim.argtypes = [vp, dp, dp, C.c_int]
+    L.hype_set_state.argtypes = [vp, dp, C.c_int]
+    L.hype_get_state.argtypes = [vp, dp, C.c_int]
+    L.hype_begin.argtypes = [vp, C.c_double, C.c_double, C.c_int, C.c_uint]
+    L.hype_advance.argtypes = [vp, C.c_int]
+    L.hype_sync.argtypes = [vp, P(Status)]
+    L.hype_dt_history.argtypes = [vp, dp, C.c_int]
+    L.hype_last_error.argtypes = [vp, dp, dp]
+    L.hype_last_error.restype = C.c_char_p
+    L.hype_stream.argtypes = [vp]
+    L.hype_stream.restype = vp
+    L.hype_advance_timed.argtypes = [vp, C.c_int, dp]
+    L.hype_initial_state.argtypes = [P(Problem), C.c_int, dp, C.c_int, C.c_ulonglong, C.c_int, dp]
+    L.hype_rankine_hugoniot.argtypes = [C.c_double, C.c_double, C.c_double, C.c_double, dp, dp, dp]
+    L.hype_rankine_hugoniot.restype = None
+    L.hype_ctx_set_tiling.argtypes = [vp, C.c_int, C.c_int]
+    L.hype_ctx_tiling.argtypes = [vp, P(C.c_int), P(C.c_int), P(C.c_int)]
+    _lib = L
+    return L
+
+
+def exported_symbols():
+    """Names declared in include/hype2d.h (checked by the CPU tests)."""
+    return [
+        "hype_nvar", "hype_ctx_create", "hype_ctx_destroy", "hype_rhs", "hype_rhs_diag",
+        "hype_cons2prim", "hype_set_state", "hype_get_state", "hype_begin", "hype_advance",
+        "hype_sync", "hype_dt_history", "hype_last_error", "hype_stream",
+        "hype_ctx_set_tiling", "hype_ctx_tiling", "hype_advance_timed", "hype_initial_state",
+        "hype_rankine_hugoniot",
+    ]
+
+
+IC_UNIFORM, IC_QUADRANTS, IC_SHOCK_BUBBLE, IC_WATER_AIR, IC_SMOOTH_VORTEX = range(5)
+COORD_EXACT, COORD_DMDA = 0, 1
+
+
+def initial_state(problem, kind, params=(), seed=0, coord=COORD_EXACT):
+    """Cell-averaged initial state (hype_initial_state), shape (ny, nx, nvar)."""
+    L = lib()
+    nv = NVAR[problem.system]
+    U = np.empty((problem.ny, problem.nx, nv), dtype=np.float64)
+    a = np.ascontiguousarray(params, dtype=np.float64)
+    if a.size == 0:
+        a = np.zeros(1)
+    rc = L.hype_initial_state(C.byref(problem), kind, _dptr(a), int(len(params)), seed, coord, _dptr(U))
+    if rc != OK:
+        raise HypeError(rc, "hype_initial_state")
+    return U
+
+
+def rankine_hugoniot(gamma, rho0, p0, mach):
+    L = lib()
+    r, u, p = C.c_double(), C.c_double(), C.c_double()
+    L.hype_rankine_hugoniot(gamma, rho0, p0, mach, C.byref(r), C.byref(u), C.byref(p))
+    return r.value, u.value, p.value
+
+
+def _dptr(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+class Context:
+    """One grid on the device (hype_ctx)."""
+
+    def __init__(self, problem: Problem):
+        self.L = lib()
+        self.problem = problem
+        self.nv = NVAR[problem.system]
+        self.shape = (problem.ny, problem.nx, self.nv)
+        h = C.c_void_p()
+        rc = self.L.hype_ctx_create(C.byref(problem), C.byref(h))
+        if rc != OK:
+            raise HypeError(rc, "hype_ctx_create")
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.hype_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _check(self, rc, what, allow_inadmissible=False):
+        if rc == OK or (allow_inadmissible and rc == E_INADMISSIBLE):
+            return rc
+        msg = what
+        if rc == E_INADMISSIBLE:
+            msg += ": " + self.last_error()[0]
+        raise HypeError(rc, msg)
+
+    def last_error(self):
+        x, y = C.c_double(), C.c_double()
+        s = self.L.hype_last_error(self.h, C.byref(x), C.byref(y))
+        return (s.decode() if s else ""), x.value, y.value
+
+    def rhs(self, U, t=0.0, fallback=False, allow_inadmissible=False):
+        """RHS of the host state U (shape ny x nx x nvar). Returns (RHS, dt[, mask], rc)."""
+        U = np.ascontiguousarray(U, dtype=np.float64).reshape(self.shape)
+        R = np.empty_like(U)
+        dt = C.c_double()
+        if fallback:
+            m = np.zeros(self.shape[:2], dtype=np.uint8)
+            rc = self.L.hype_rhs_diag(self.h, t, _dptr(U), _dptr(R), C.byref(dt),
+                                      m.ctypes.data_as(C.POINTER(C.c_ubyte)), MEM_HOST)
+            self._check(rc, "hype_rhs_diag", allow_inadmissible)
+            return R, dt.value, m, rc
+        rc = self.L.hype_rhs(self.h, t, _dptr(U), _dptr(R), C.byref(dt), MEM_HOST)
+        self._check(rc, "hype_rhs", allow_inadmissible)
+        return R, dt.value, rc
+
+    def rhs_device(self, U_ptr, R_ptr, t=0.0):
+        dt = C.c_double()
+        rc = self.L.hype_rhs(self.h, t, C.cast(U_ptr, C.POINTER(C.c_double)),
+                             C.cast(R_ptr, C.POINTER(C.c_double)), C.byref(dt), MEM_DEVICE)
+        self._check(rc, "hype_rhs(device)")
+        return dt.value
+
+    def cons2prim(self, U):
+        U = np.ascontiguousarray(U, dtype=np.float64).reshape(self.shape)
+        W = np.empty_like(U)
+        self._check(self.L.hype_cons2prim(self.h, _dptr(U), _dptr(W), MEM_HOST), "hype_cons2prim")
+        return W
+
+    def set_state(self, U):
+        U = np.ascontiguousarray(U, dtype=np.float64).reshape(self.shape)
+        self._check(self.L.hype_set_state(self.h, _dptr(U), MEM_HOST), "hype_set_state")
+
+    def get_state(self):
+        U = np.empty(self.shape, dtype=np.float64)
+        self._check(self.L.hype_get_state(self.h, _dptr(U), MEM_HOST), "hype_get_state")
+        return U
+
+    def begin(self, t0=0.0, t_end=float("inf"), max_steps=1 << 30, flags=TS_CLIP):
+        self._check(self.L.hype_begin(self.h, t0, t_end, max_steps, flags), "hype_begin")
+
+    def advance(self, nsteps):
+        self._check(self.L.hype_advance(self.h, nsteps), "hype_advance")
+
+    def sync(self, allow_inadmissible=False):
+        st = Status()
+        rc = self.L.hype_sync(self.h, C.byref(st))
+        self._check(rc, "hype_sync", allow_inadmissible)
+        return st
+
+    def dt_history(self, n):
+        d = np.zeros(n, dtype=np.float64)
+        self._check(self.L.hype_dt_history(self.h, _dptr(d), n), "hype_dt_history")
+        return d
+
+    def advance_timed(self, nsteps):
+        ms = C.c_double()
+        self._check(self.L.hype_advance_timed(self.h, nsteps, C.byref(ms)), "hype_advance_timed")
+        return ms.value
+
+    def set_tiling(self, hrows, grid=0):
+        self._check(self.L.hype_ctx_set_tiling(self.h, hrows, grid), "hype_ctx_set_tiling")
+
+    def tiling(self):
+        a, b, c = C.c_int(), C.c_int(), C.c_int()
+        self._check(self.L.hype_ctx_tiling(self.h, C.byref(a), C.byref(b), C.byref(c)), "hype_ctx_tiling")
+        return a.value, b.value, c.value
+
+    def solve(self, U0, steps, t0=0.0, t_end=float("inf"), flags=TS_CLIP, chunk=None):
+        """Advance U0 by `steps` steps (or to t_end); returns (U, status)."""
+        self.set_state(U0)
+        self.begin(t0, t_end, steps, flags)
+        self.advance(steps)
+        st = self.sync()
+        return self.get_state(), st

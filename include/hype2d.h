@@ -1,0 +1,194 @@
+/*
+ * hype2d.h -- C ABI of the MI355X-native HyPE-2D finite-volume right-hand side.
+ *
+ * The device RHS is a drop-in for the reference's PETSc TSRHSFunction callbacks
+ *   PetscErrorCode RHSFunctionPrim(TS, PetscReal t, Vec U, Vec RHS, void *ctx)
+ *   PetscErrorCode RHSFunction    (TS, PetscReal t, Vec U, Vec RHS, void *ctx)
+ * (HyPE-2D/hype.h:305-306, rhs_function_prim.c:12, rhs_function.c:12) on the
+ * same DMDA cell-centred layout: AoS records of nvar doubles, row-major
+ * [j][i][c] over the owned nx*ny cells (hype.h:68-70, hype.c:108-120).  The
+ * ghost frame of width 3 is synthesised on the device from the boundary kinds
+ * (replacing DMGlobalToLocal and the boundary loops, rhs_function_prim.c:44-315).
+ *
+ * All arithmetic is IEEE FP64.  Plain pointers and sizes only; no HIP or torch
+ * types appear in this header.  One context per host thread; device work of a
+ * context runs on its own HIP stream.
+ */
+#ifndef HYPE2D_H
+#define HYPE2D_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---------------------------------------------------- */
+#define HYPE_OK 0
+#define HYPE_E_INVALID 1      /* bad argument or problem description */
+#define HYPE_E_UNSUPPORTED 2  /* combination the reference leaves undefined */
+#define HYPE_E_DEVICE 3       /* HIP runtime failure */
+#define HYPE_E_NOMEM 4        /* device or host allocation failed */
+#define HYPE_E_INADMISSIBLE 5 /* replaces MPI_Abort on an inadmissible state
+                                 (kapila_5eqn.c:95-99,353-357; euler.c:60-70,
+                                 212-222; baer_nunziato.c:91-113,281-303) */
+#define HYPE_E_BUSY 6         /* server: no slot or HBM for the request yet */
+#define HYPE_E_NOTFOUND 7     /* server: unknown request id */
+
+/* ---- PDE systems (compile-time #define in hype.h:33-35, runtime here) -- */
+#define HYPE_EULER 0          /* (rho, rho u, rho v, E), nvar 4 (euler.c) */
+#define HYPE_KAPILA 1         /* (a1 r1, a2 r2, rho u, rho v, E, a1), nvar 6 */
+#define HYPE_BAER_NUNZIATO 2  /* (p1 r1, p1 r1 u1, p1 r1 v1, p1 E1,
+                                 p2 r2, ..., p1), nvar 9 */
+
+/* ---- boundary kinds, same values as enum bndry_type (hype.h:76) ------- */
+#define HYPE_BC_WALL 0         /* no implementation in the reference: rejected */
+#define HYPE_BC_INLET 1        /* hard-coded jet profile: rejected */
+#define HYPE_BC_PERIODIC 2
+#define HYPE_BC_REFLECTIVE 3
+#define HYPE_BC_TRANSMISSIVE 4
+
+/* ---- quirk switches ---------------------------------------------------- */
+/* baer_nunziato.c:271 reads p_2 = V[8] (the volume fraction) in the
+ * primitive-path flux.  Set: reproduce it (reference parity).  Clear: p_2 =
+ * V[7] (physical). */
+#define HYPE_QUIRK_BN_P2_IS_PHI 0x1u
+
+/* ---- memory location of caller buffers -------------------------------- */
+#define HYPE_MEM_HOST 0
+#define HYPE_MEM_DEVICE 1
+
+/* ---- problem description: the fields of AppCtx the RHS reads
+ *      (hype.h:203-252) ------------------------------------------------- */
+typedef struct hype_problem {
+    int system;            /* HYPE_EULER / HYPE_KAPILA / HYPE_BAER_NUNZIATO */
+    int order;             /* AppCtx.N, 0..2 (reconstruction degree) */
+    int recons_prim;       /* AppCtx.ReconsPrimitive: 1 RHSFunctionPrim, 0 RHSFunction */
+    int nx, ny;            /* AppCtx.N_x, N_y (>= 3) */
+    double x_min, x_max;   /* h = (x_max - x_min)/nx for both axes (hype.c:57) */
+    double y_min, y_max;
+    double cfl;            /* AppCtx.CFL */
+    int bc[4];             /* left, right, bottom, top (HYPE_BC_*) */
+    double gamma1, gamma2; /* GAMMA_1, GAMMA_2 (hype.h:42-43); Euler uses gamma1 */
+    double pi1, pi2;       /* PI_1, PI_2 (hype.h:44-45) */
+    unsigned quirks;       /* HYPE_QUIRK_* */
+} hype_problem;
+
+/* number of components of a system (the reference's nVar, hype.h:37) */
+int hype_nvar(int system);
+
+/* ======================================================================
+ * Single-instance context
+ * ====================================================================== */
+typedef struct hype_ctx hype_ctx;
+
+/* Allocates the device state for one grid (replaces DMDACreate2d + the
+ * scratch of hype.c:108-128,235-251).  Returns HYPE_OK or an error code. */
+int hype_ctx_create(const hype_problem *p, hype_ctx **out);
+void hype_ctx_destroy(hype_ctx *c);
+
+/* The RHS callback (rhs_function_prim.c:12-626 / rhs_function.c:12-625).
+ * U and RHS: nx*ny*nvar doubles, AoS [j][i][c], both host or both device
+ * (mem = HYPE_MEM_*).  *dt_out (may be NULL) receives CFL*h/(2*s_max), the
+ * Ctx->dt side effect (rhs_function_prim.c:620-623).  t is accepted for
+ * signature parity; the supported boundaries do not depend on it.
+ * Returns HYPE_E_INADMISSIBLE where the reference would call MPI_Abort; the
+ * RHS is still written and hype_last_error() describes the first offending
+ * face (lowest face index). */
+int hype_rhs(hype_ctx *c, double t, const double *U, double *RHS,
+             double *dt_out, int mem);
+
+/* Same, plus diagnostics: fallback (nx*ny bytes, host) receives 1 for every
+ * owned cell whose reconstruction failed the admissibility check and fell
+ * back to minmod slopes (rhs_function_prim.c:393-421). */
+int hype_rhs_diag(hype_ctx *c, double t, const double *U, double *RHS,
+                  double *dt_out, unsigned char *fallback, int mem);
+
+/* Primitive pass (ComputePrimitiveVariables, post_process.c:165-200). */
+int hype_cons2prim(hype_ctx *c, const double *U, double *W, int mem);
+
+/* ---- device-resident time integration -------------------------------- */
+/* Stepping options */
+#define HYPE_TS_CLIP 0x1u      /* shorten the step that would pass t_end
+                                  (TS_EXACTFINALTIME_MATCHSTEP, hype.c:359) */
+#define HYPE_TS_GRAPH 0x2u     /* replay captured hipGraphs per step */
+
+typedef struct hype_status {
+    double t;                  /* simulated time reached */
+    double dt;                 /* dt of the last step taken */
+    int steps;                 /* steps taken since hype_begin */
+    int done;                  /* t >= t_end or steps >= max_steps */
+    int err;                   /* HYPE_OK or HYPE_E_INADMISSIBLE */
+} hype_status;
+
+/* Copy a state in/out of the context (same layout as hype_rhs). */
+int hype_set_state(hype_ctx *c, const double *U, int mem);
+int hype_get_state(hype_ctx *c, double *U, int mem);
+
+/* Start a solve on the current state: priming RHS call for dt (hype.c:343,
+ * 356), time t0, stop at t_end or after max_steps steps.  The scheme is the
+ * one the reference selects for order N (hype.c:362-382): N=0 forward Euler,
+ * N=1 SSPRK(2,2), N=2 SSPRK(4,3).  dt of step k+1 comes from the last RHS
+ * evaluation of step k (MonitorFunction, post_process.c:19), computed and
+ * consumed on the device. */
+int hype_begin(hype_ctx *c, double t0, double t_end, int max_steps,
+               unsigned flags);
+
+/* Enqueue nsteps steps (asynchronous; steps past t_end are no-ops). */
+int hype_advance(hype_ctx *c, int nsteps);
+
+/* hype_advance bracketed by HIP events on the context stream; *ms receives
+ * the device time of the nsteps steps (waits for them to finish). */
+int hype_advance_timed(hype_ctx *c, int nsteps, double *ms);
+
+/* Wait for queued work and report progress. */
+int hype_sync(hype_ctx *c, hype_status *st);
+
+/* dt of every step taken so far (up to n entries). */
+int hype_dt_history(hype_ctx *c, double *dts, int n);
+
+/* Human-readable description of the last inadmissible state, with the face
+ * coordinates the reference prints (x_min + i*h, y_min + j*h). */
+const char *hype_last_error(hype_ctx *c, double *x, double *y);
+
+/* Device stream of the context as an opaque handle (hipStream_t). */
+void *hype_stream(hype_ctx *c);
+
+/* ======================================================================
+ * Initial states: cell averages by 5x5 Gauss-Legendre quadrature of the
+ * pointwise state in conserved variables (InitializeSolution,
+ * initialize_solution.c:21-81).
+ * ====================================================================== */
+#define HYPE_IC_UNIFORM 0       /* params: the nvar primitive values */
+#define HYPE_IC_QUADRANTS 1     /* Euler: xs, ys, then (rho,u,v,p) of NE, NW, SW, SE */
+#define HYPE_IC_SHOCK_BUBBLE 2  /* Kapila: mach, x_shock, cx, cy, R, eps, width,
+                                   rho_air, rho_he, p0 (air = phase 1) */
+#define HYPE_IC_WATER_AIR 3     /* Baer-Nunziato: y_iface, eps, width, cx, cy,
+                                   r_burst, p_burst, p0, rho_water, rho_air,
+                                   nmodes (<= 8), amplitude, length; mode phases
+                                   drawn from seed (water = phase 1, below) */
+#define HYPE_IC_SMOOTH_VORTEX 4 /* Kapila SmoothVortex (kapila_5eqn.c:527-562) */
+
+#define HYPE_COORD_EXACT 0      /* cell centres x_min + (i+1/2) h */
+#define HYPE_COORD_DMDA 1       /* the centres DMDASetUniformCoordinates gives the
+                                   reference (hype.c:132-135) */
+
+int hype_initial_state(const hype_problem *p, int kind, const double *params,
+                       int nparams, unsigned long long seed, int coord_mode,
+                       double *U);
+
+/* Post-shock state of a Mach-M shock moving +x into gas at rest
+ * (Rankine-Hugoniot, ideal/stiffened gas with zero stiffness). */
+void hype_rankine_hugoniot(double gamma, double rho0, double p0, double M,
+                           double *rho1, double *u1, double *p1);
+
+/* ---- tiling (used by the autotuner and the tests) ------------------- */
+/* Rows per tile of the line-sweep kernel and persistent grid size (0 = one
+ * workgroup per tile).  Results do not depend on either. */
+int hype_ctx_set_tiling(hype_ctx *c, int hrows, int grid);
+int hype_ctx_tiling(hype_ctx *c, int *hrows, int *ntiles, int *grid);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* HYPE2D_H */
