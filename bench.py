@@ -88,6 +88,29 @@ def cpu_baseline(p, U0, nv, budget_s=12.0):
                       f"differencing a 1-step and a {k + 1}-step run)"}
 
 
+def calibrate_copy(mib):
+    """A plain copy kernel of known size: its FETCH_SIZE / WRITE_SIZE counters
+    per dispatch calibrate the stage kernel's traffic figures."""
+    import torch
+
+    n = mib * (1 << 20) // 8
+    a = torch.ones(n, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    reps = 10
+    e0.record()
+    for _ in range(reps):
+        b.copy_(a)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    print(json.dumps({"copy_mib": mib, "bytes_per_copy": 2 * n * 8, "ms": ms,
+                      "GBps": 2 * n * 8 / ms / 1e6}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -97,7 +120,12 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--graph", type=int, default=1)
     ap.add_argument("--hrows", type=int, default=0)
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="run only a plain device copy of MIB MiB (counter calibration)")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        calibrate_copy(args.calibrate_copy)
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is a design only "
                                    "(see DESIGN.md), so --gpus must be 1"}))

@@ -12,6 +12,7 @@
 
 namespace hype {
 hipError_t launch_stage(const hype_launch& L, hipStream_t s);
+int blocks_per_cu(const hype_variant& v);
 hipError_t launch_cons2prim(int system, const double* U, double* W, long n, Eos e, hipStream_t s);
 }  // namespace hype
 
@@ -145,6 +146,8 @@ int hyped_num_cus(void) {
   return n;
 }
 
+int hyped_blocks_per_cu(const hype_variant* v) { return hype::blocks_per_cu(*v); }
+
 int hyped_graph_begin(void* s) {
   HCHECK(hipStreamBeginCapture((hipStream_t)s, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
   return HYPE_OK;
@@ -168,6 +171,18 @@ int hyped_graph_launch(void* exec, void* s) {
 
 int hyped_graph_destroy(void* exec) {
   if (exec) HCHECK(hipGraphExecDestroy((hipGraphExec_t)exec), "hipGraphExecDestroy");
+  return HYPE_OK;
+}
+
+int hyped_stream_wait(void* s, void* ev) {
+  HCHECK(hipStreamWaitEvent((hipStream_t)s, (hipEvent_t)ev, 0), "hipStreamWaitEvent");
+  return HYPE_OK;
+}
+
+int hyped_event_create_sync(void** ev) {
+  hipEvent_t e;
+  HCHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreateWithFlags");
+  *ev = (void*)e;
   return HYPE_OK;
 }
 

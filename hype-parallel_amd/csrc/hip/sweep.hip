@@ -1,10 +1,10 @@
 // sweep.hip -- the fused finite-volume stage kernel for CDNA4 (gfx950).
 //
-// One wavefront owns a tile: a strip of HYPE_TILE_W (62) owned columns plus
-// one ring column on each side, and `hrows` owned rows.  Lane l works on
+// One wavefront owns a tile: a strip of up to HYPE_TILE_W (62) owned columns
+// plus one ring column on each side, and h owned rows.  Lane l works on
 // column x0-1+l and the wave marches up the strip one row per iteration:
 //
-//   * a rolling window of 6 rows x 68 columns of the stage state lives in
+//   * a rolling window of 5 rows x 68 columns of the stage state lives in
 //     LDS (component-major, so neighbouring lanes read neighbouring doubles).
 //     Each iteration loads one new row, synthesises ghost cells from the
 //     boundary index maps (the device replacement for DMGlobalToLocal and the
@@ -107,7 +107,7 @@ __device__ __forceinline__ unsigned long long wave_min_u64(unsigned long long v)
   return v;
 }
 
-constexpr int NSLOT = 6;
+constexpr int NSLOT = 5;
 constexpr int COLS = HYPE_WAVE + 4;
 
 template <class S, bool PRIM, int N>
@@ -116,33 +116,33 @@ struct Tile {
   using Qd = Quad<N>;
   static constexpr int NDOF = Qd::NDOF, NGF = Qd::NGF, NGV = Qd::NGV, NNODE = Qd::NNODE;
 
-  // values at the face points and the volume points of one cell from its
-  // modal coefficients (rhs_function_prim.c:382-388, 425-456)
-  __device__ static __forceinline__ void eval_points(const double* co, int c, double (&fv)[NNODE][NV],
-                                                     double (&vv)[NGV][NV], double (&vgx)[NGV][NV],
-                                                     double (&vgy)[NGV][NV], double r1_h) {
-#pragma unroll
-    for (int p = 0; p < NNODE; ++p) {
+  // value of component c at node/face point p from its modal coefficients
+  // (rhs_function_prim.c:382-388, 425-440)
+  __device__ static __forceinline__ double at_node(const double (&co)[NV][NDOF], int c, int p) {
+    double v;
 #define PHIN(k) Qd::phin(p, k)
-      HYPE_BASIS_SUM(fv[p][c], co, NDOF, PHIN);
+    HYPE_BASIS_SUM(v, co[c], NDOF, PHIN);
 #undef PHIN
-    }
-    if constexpr (S::HAS_NCP) {
+    return v;
+  }
+  // value and scaled gradient at volume point q (rhs_function_prim.c:442-455)
+  __device__ static __forceinline__ void at_vol(const double (&co)[NV][NDOF], int q, double r1_h, double* V,
+                                                double* gx, double* gy) {
 #pragma unroll
-      for (int q = 0; q < NGV; ++q) {
-        double gx, gy;
+    for (int c = 0; c < NV; ++c) {
+      double a, b, d;
 #define PHIV(k) Qd::phiv(q, k)
 #define GPX(k) Qd::gphx(q, k)
 #define GPY(k) Qd::gphy(q, k)
-        HYPE_BASIS_SUM(vv[q][c], co, NDOF, PHIV);
-        HYPE_BASIS_SUM(gx, co, NDOF, GPX);
-        HYPE_BASIS_SUM(gy, co, NDOF, GPY);
+      HYPE_BASIS_SUM(a, co[c], NDOF, PHIV);
+      HYPE_BASIS_SUM(b, co[c], NDOF, GPX);
+      HYPE_BASIS_SUM(d, co[c], NDOF, GPY);
 #undef PHIV
 #undef GPX
 #undef GPY
-        vgx[q][c] = r1_h * gx;
-        vgy[q][c] = r1_h * gy;
-      }
+      V[c] = a;
+      gx[c] = r1_h * b;
+      gy[c] = r1_h * d;
     }
   }
 
@@ -198,8 +198,7 @@ struct Tile {
 
     // ---- geometry ------------------------------------------------------
     const int x0 = T.x0, y0 = T.y0;
-    const int W = min(HYPE_TILE_W, nx - x0);
-    const int H = min(I->hrows, ny - y0);
+    const int W = T.w, H = T.h;
     const int col = x0 - 1 + lane;
     const bool owned = lane >= 1 && lane <= W;
     const bool xlane = lane >= 1 && lane <= W + 1;
@@ -279,11 +278,11 @@ struct Tile {
       const bool doAsm = (r >= y0 + 1);
       const int sm2 = slot(r - 2), sm1 = slot(r - 1), s0 = slot(r), sp1 = slot(r + 1), sp2 = slot(r + 2);
 
-      // ---- reconstruction ----------------------------------------------
-      double fv[NNODE][NV], vv[NGV][NV], vgx[NGV][NV], vgy[NGV][NV];
+      // ---- reconstruction: modal coefficients of every component --------
+      double co[NV][NDOF];
 #pragma unroll
       for (int c = 0; c < NV; ++c) {
-        double ux[5], uy[5], ud[5], co[NDOF];
+        double ux[5], uy[5], ud[5];
         ux[0] = W_(s0, c, cw - 2);
         ux[1] = W_(s0, c, cw - 1);
         ux[2] = W_(s0, c, cw);
@@ -299,28 +298,39 @@ struct Tile {
         ud[2] = W_(sm1, c, cw + 1);
         ud[3] = W_(sp1, c, cw - 1);
         ud[4] = W_(sm1, c, cw - 1);
-        reconstruct<N>(ux, uy, ud, co);
-        eval_points(co, c, fv, vv, vgx, vgy, r1_h);
+        reconstruct<N>(ux, uy, ud, co[c]);
       }
+      // admissibility at the nodes (rhs_function_prim.c:393-404)
       bool pad = true;
 #pragma unroll
-      for (int p = 0; p < NNODE; ++p) pad = pad && (PRIM ? S::pad_prim(eos, fv[p]) : S::pad(eos, fv[p]));
+      for (int p = 0; p < NNODE; ++p) {
+        double V[NV];
+#pragma unroll
+        for (int c = 0; c < NV; ++c) V[c] = at_node(co, c, p);
+        pad = pad && (PRIM ? S::pad_prim(eos, V) : S::pad(eos, V));
+      }
       if (!pad) {  // rhs_function_prim.c:407-421 / rhs_function.c:409-421
 #pragma unroll
         for (int c = 0; c < NV; ++c) {
-          double co[NDOF];
           const double w0 = W_(s0, c, cw);
-          co[0] = w0;
+          co[c][0] = w0;
           if constexpr (NDOF > 1) {
-            co[1] = PRIM ? minmod(W_(s0, c, cw + 1) - w0, w0 - W_(s0, c, cw - 1)) : 0.0;
-            co[2] = PRIM ? minmod(W_(sp1, c, cw) - w0, w0 - W_(sm1, c, cw)) : 0.0;
+            co[c][1] = PRIM ? minmod(W_(s0, c, cw + 1) - w0, w0 - W_(s0, c, cw - 1)) : 0.0;
+            co[c][2] = PRIM ? minmod(W_(sp1, c, cw) - w0, w0 - W_(sm1, c, cw)) : 0.0;
           }
 #pragma unroll
-          for (int k = 3; k < NDOF; ++k) co[k] = 0.0;
-          eval_points(co, c, fv, vv, vgx, vgy, r1_h);
+          for (int k = 3; k < NDOF; ++k) co[c][k] = 0.0;
         }
       }
       if (L.fallback && ownedRow && owned) L.fallback[(size_t)r * nx + col] = (unsigned char)(!pad);
+
+      // ---- slide the window: row r-2 is no longer read, row r+3 takes its
+      // slot (the wave's own reads above precede this store in program order)
+      if (more) {
+        const int sn = slot(r + 3);
+        put(sn, lane, qa);
+        if (lane < 4) put(sn, lane + HYPE_WAVE, qb);
+      }
 
       // ---- volume non-conservative product (rhs_function_prim.c:461-479)
       double ncp[NV];
@@ -330,14 +340,22 @@ struct Tile {
         if (mode != HYPE_MODE_PRIME && ownedRow) {
 #pragma unroll
           for (int q = 0; q < NGV; ++q) {
-            double B[NV];
-            S::template vol_ncp<PRIM>(eos, vv[q], vgx[q], vgy[q], B);
+            double V[NV], gx[NV], gy[NV], B[NV];
+            at_vol(co, q, r1_h, V, gx, gy);
+            S::template vol_ncp<PRIM>(eos, V, gx, gy, B);
 #pragma unroll
             for (int c = 0; c < NV; ++c)
               if (S::DNZ[c]) ncp[c] = ncp[c] + -WV * B[c];
           }
         }
       }
+
+      // ---- values at the face quadrature points (u_bnd, :425-440) -------
+      double fv[NNODE][NV];
+#pragma unroll
+      for (int p = 0; p < NNODE; ++p)
+#pragma unroll
+        for (int c = 0; c < NV; ++c) fv[p][c] = at_node(co, c, p);
 
       // ---- x-faces i-1/2 (rhs_function_prim.c:485-533) -------------------
       double xpart[NV];
@@ -450,13 +468,7 @@ struct Tile {
         for (int q = 0; q < NGF; ++q) topPrev[q][c] = fv[Qd::T(q)][c];
       }
 
-      // ---- slide the window ------------------------------------------------
-      if (more) {
-        const int sn = slot(r + 3);
-        put(sn, lane, qa);
-        if (lane < 4) put(sn, lane + HYPE_WAVE, qb);
-      }
-      __syncthreads();
+      __syncthreads();  // the new row is visible to the next iteration
     }
 
     // ---- CFL maximum and error word ----------------------------------------
@@ -482,15 +494,16 @@ template <class S, bool PRIM, int N>
 __global__ __launch_bounds__(HYPE_WAVE) void stage_kernel(hype_launch L) {
   __shared__ double win[NSLOT * S::NV * COLS];
   const int lane = threadIdx.x;
+  const int ntiles = L.ntiles_dev ? *L.ntiles_dev : L.ntiles;
   if (L.grid <= 0) {
-    if ((int)blockIdx.x < L.ntiles) Tile<S, PRIM, N>::run(L, blockIdx.x, win, lane);
+    if ((int)blockIdx.x < ntiles) Tile<S, PRIM, N>::run(L, blockIdx.x, win, lane);
     return;
   }
   for (;;) {
     unsigned t = 0;
     if (lane == 0) t = atomicAdd(&L.queue[0], 1u);
     const int tile = __builtin_amdgcn_readfirstlane((int)__shfl(t, 0, HYPE_WAVE));
-    if (tile >= L.ntiles) break;
+    if (tile >= ntiles) break;
     Tile<S, PRIM, N>::run(L, tile, win, lane);
     __syncthreads();
   }
@@ -506,7 +519,7 @@ __global__ __launch_bounds__(HYPE_WAVE) void stage_kernel(hype_launch L) {
 
 template <class S, bool PRIM, int N>
 static hipError_t launch_variant(const hype_launch& L, hipStream_t s) {
-  const int grid = L.grid > 0 ? L.grid : L.ntiles;
+  const int grid = L.grid > 0 ? L.grid : L.ntiles;  // device-resident counts need grid > 0
   hipLaunchKernelGGL((stage_kernel<S, PRIM, N>), dim3(grid), dim3(HYPE_WAVE), 0, s, L);
   return hipGetLastError();
 }
@@ -521,8 +534,34 @@ static hipError_t launch_sys(const hype_launch& L, hipStream_t s) {
   }
 }
 
+template <class S, bool PRIM, int N>
+static int occupancy_variant() {
+  int blocks = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, stage_kernel<S, PRIM, N>, HYPE_WAVE, 0) != hipSuccess)
+    return 0;
+  return blocks;
+}
+
+template <class S>
+static int occupancy_sys(const hype_variant& v) {
+  const bool p = v.prim != 0;
+  switch (v.order) {
+    case 0: return p ? occupancy_variant<S, true, 0>() : occupancy_variant<S, false, 0>();
+    case 1: return p ? occupancy_variant<S, true, 1>() : occupancy_variant<S, false, 1>();
+    default: return p ? occupancy_variant<S, true, 2>() : occupancy_variant<S, false, 2>();
+  }
+}
+
+int blocks_per_cu(const hype_variant& v) {
+  switch (v.system) {
+    case HYPE_EULER: return occupancy_sys<Euler>(v);
+    case HYPE_KAPILA: return occupancy_sys<Kapila>(v);
+    default: return occupancy_sys<BN>(v);
+  }
+}
+
 hipError_t launch_stage(const hype_launch& L, hipStream_t s) {
-  if (L.ntiles <= 0) return hipSuccess;
+  if (L.ntiles <= 0 && !L.ntiles_dev) return hipSuccess;
   switch (L.v.system) {
     case HYPE_EULER: return launch_sys<Euler>(L, s);
     case HYPE_KAPILA: return launch_sys<Kapila>(L, s);

@@ -36,40 +36,53 @@ int hype_check_problem(const hype_problem *p, int *nvar)
     return HYPE_OK;
 }
 
-int hype_count_tiles(int nx, int ny, int hrows)
-{
-    int tx = (nx + HYPE_TILE_W - 1) / HYPE_TILE_W;
-    int ty = (ny + hrows - 1) / hrows;
-    return tx * ty;
-}
+/* near-equal split of n into k parts: part i is [i*n/k, (i+1)*n/k) */
+static int part_lo(int i, int n, int k) { return (int)((long)i * n / k); }
 
-int hype_make_tiles(int inst, int nx, int ny, int hrows, hype_tile *tiles, int cap)
+int hype_make_tiles(int inst, int nx, int ny, int nsx, int nsy, hype_tile *tiles, int cap)
 {
-    int n = 0, x0, y0;
-    for (y0 = 0; y0 < ny; y0 += hrows)
-        for (x0 = 0; x0 < nx; x0 += HYPE_TILE_W) {
+    int n = 0, a, b;
+    if (nsx < 1 || nsy < 1 || nsx > nx || nsy > ny) return -1;
+    for (b = 0; b < nsy; ++b)
+        for (a = 0; a < nsx; ++a) {
             if (n >= cap) return -1;
             tiles[n].inst = inst;
-            tiles[n].x0 = x0;
-            tiles[n].y0 = y0;
+            tiles[n].x0 = part_lo(a, nx, nsx);
+            tiles[n].w = part_lo(a + 1, nx, nsx) - tiles[n].x0;
+            tiles[n].y0 = part_lo(b, ny, nsy);
+            tiles[n].h = part_lo(b + 1, ny, nsy) - tiles[n].y0;
             tiles[n].flags = (n == 0) ? 1 : 0;
+            if (tiles[n].w > HYPE_TILE_W || tiles[n].w < 1 || tiles[n].h < 1) return -1;
             ++n;
         }
     return n;
 }
 
-/* Rows per tile.  A tile costs (hrows+2) reconstructed rows for hrows owned
- * ones, so tall tiles waste less; but the grid must offer enough tiles to
- * keep every SIMD of the device busy (at least ~4 waves per CU). */
-int hype_default_rows(int nx, int ny, int system, int ncu)
+int hype_strips(int nx) { return (nx + HYPE_TILE_W - 1) / HYPE_TILE_W; }
+
+/* Tile-shape model.  A wave sweeps its strip one row per iteration with all
+ * 64 lanes busy whatever the strip width, so a tile of h owned rows costs
+ * about (h + 2 + c0) row-iterations (two ring rows, c0 for the 5-row
+ * prologue).  Tiles run `slots` at a time (resident waves of the device), so
+ * a launch of T tiles takes ceil(T/slots) rounds of the tallest tile.
+ * Returns the number of row segments minimising rounds x tile cost. */
+int hype_pick_segments(int nx, int ny, int slots, int extra_tiles)
 {
-    int tx = (nx + HYPE_TILE_W - 1) / HYPE_TILE_W;
-    int want = (ncu > 0 ? ncu : 256) * 4;
-    int h = 64;
-    (void)system;
-    while (h > 8 && (long)tx * ((ny + h - 1) / h) < want) h /= 2;
-    if (h > ny) h = ny;
-    return h;
+    const double c0 = 1.0;
+    int nsx = hype_strips(nx), k, best = 1;
+    double best_cost = 1e300;
+    if (slots < 1) slots = 1;
+    for (k = 1; k <= ny; ++k) {
+        long tiles = (long)nsx * k + extra_tiles;
+        long rounds = (tiles + slots - 1) / slots;
+        int hmax = (ny + k - 1) / k;
+        double cost = (double)rounds * (hmax + 2.0 + c0);
+        if (cost < best_cost - 1e-9) {
+            best_cost = cost;
+            best = k;
+        }
+    }
+    return best;
 }
 
 void hype_decode_error(unsigned long long word, int nx, int ny, int ngf,
@@ -106,15 +119,15 @@ void hype_decode_error(unsigned long long word, int nx, int ny, int ngf,
     snprintf(msg, cap, "%s\nAt x = %f, y = %f", head, *x, *y);
 }
 
-void hype_fill_inst(hype_inst *I, const hype_problem *p, int hrows)
+void hype_fill_inst(hype_inst *I, const hype_problem *p, int nsy)
 {
     int s;
     memset(I, 0, sizeof(*I));
     I->nx = p->nx;
     I->ny = p->ny;
     for (s = 0; s < 4; ++s) I->bc[s] = p->bc[s];
-    I->hrows = hrows;
-    I->ntiles_x = (p->nx + HYPE_TILE_W - 1) / HYPE_TILE_W;
+    I->nsx = hype_strips(p->nx);
+    I->nsy = nsy;
     I->h = (p->x_max - p->x_min) / (double)(p->nx); /* hype.c:57 */
     I->r1_h = 1. / (I->h);                          /* rhs_function_prim.c:26 */
     I->cfl = p->cfl;

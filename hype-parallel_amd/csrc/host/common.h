@@ -14,13 +14,17 @@
 /* validate a problem description; fills nvar */
 int hype_check_problem(const hype_problem *p, int *nvar);
 
-/* default rows per tile for a grid, given the device's CU count */
-int hype_default_rows(int nx, int ny, int system, int ncu);
+/* strips of width <= HYPE_TILE_W covering nx columns */
+int hype_strips(int nx);
 
-/* build the tile list of one instance into tiles (capacity cap); returns
- * the number of tiles (first one flagged as leader) */
-int hype_make_tiles(int inst, int nx, int ny, int hrows, hype_tile *tiles, int cap);
-int hype_count_tiles(int nx, int ny, int hrows);
+/* number of row segments for a grid on a device with `slots` resident
+ * waves of the kernel (tile-shape model in common.c) */
+int hype_pick_segments(int nx, int ny, int slots, int extra_tiles);
+
+/* build the tile list of one instance (nsx strips x nsy segments, near-equal
+ * splits) into tiles (capacity cap); returns the number of tiles, the first
+ * flagged as leader, or -1 */
+int hype_make_tiles(int inst, int nx, int ny, int nsx, int nsy, hype_tile *tiles, int cap);
 
 /* decode the packed error word into a message and face coordinates */
 void hype_decode_error(unsigned long long word, int nx, int ny, int ngf,
@@ -28,6 +32,6 @@ void hype_decode_error(unsigned long long word, int nx, int ny, int ngf,
                        size_t cap, double *x, double *y);
 
 /* fill the device-independent fields of an instance descriptor */
-void hype_fill_inst(hype_inst *I, const hype_problem *p, int hrows);
+void hype_fill_inst(hype_inst *I, const hype_problem *p, int nsy);
 
 #endif

@@ -16,7 +16,7 @@
 
 struct hype_ctx {
     hype_problem P;
-    int nv, nstages, hrows, ntiles, grid;
+    int nv, nstages, nsy, ntiles, grid, slots;
     size_t ncell, nbytes;
     void *stream;
     double *buf[3];         /* U, A, B (device) */
@@ -69,8 +69,16 @@ int hype_ctx_create(const hype_problem *p, hype_ctx **out)
     c->nstages = hyped_num_stages(p->order);
     c->ncell = (size_t)p->nx * p->ny;
     c->nbytes = c->ncell * nv * sizeof(double);
-    c->hrows = hype_default_rows(p->nx, p->ny, p->system, hyped_num_cus());
-    c->ntiles = hype_count_tiles(p->nx, p->ny, c->hrows);
+    {
+        hype_variant v;
+        v.system = p->system;
+        v.prim = p->recons_prim;
+        v.order = p->order;
+        c->slots = hyped_num_cus() * hyped_blocks_per_cu(&v);
+        if (c->slots < 1) c->slots = 1024;
+    }
+    c->nsy = hype_pick_segments(p->nx, p->ny, c->slots, 0);
+    c->ntiles = hype_strips(p->nx) * c->nsy;
     c->grid = 0;
     if ((rc = hyped_stream_create(&c->stream)) != HYPE_OK) goto fail;
     for (i = 0; i < 3; ++i)
@@ -82,7 +90,7 @@ int hype_ctx_create(const hype_problem *p, hype_ctx **out)
     if ((rc = hyped_malloc((void **)&c->queue_d, 4 * sizeof(unsigned))) != HYPE_OK) goto fail;
     if ((rc = hyped_fill_u64(c->queue_d, 0ull, 2, c->stream)) != HYPE_OK) goto fail;
 
-    hype_fill_inst(&c->inst_h, p, c->hrows);
+    hype_fill_inst(&c->inst_h, p, c->nsy);
     c->inst_h.buf[0] = c->buf[0];
     c->inst_h.buf[1] = c->buf[1];
     c->inst_h.buf[2] = c->buf[2];
@@ -91,8 +99,10 @@ int hype_ctx_create(const hype_problem *p, hype_ctx **out)
     if ((rc = hyped_memcpy_h2d(c->inst_d, &c->inst_h, sizeof(hype_inst), c->stream)) != HYPE_OK) goto fail;
     tiles = (hype_tile *)malloc((size_t)c->ntiles * sizeof(hype_tile));
     if (!tiles) { rc = HYPE_E_NOMEM; goto fail; }
-    hype_make_tiles(0, p->nx, p->ny, c->hrows, tiles, c->ntiles);
-    rc = hyped_memcpy_h2d(c->tiles_d, tiles, (size_t)c->ntiles * sizeof(hype_tile), c->stream);
+    if (hype_make_tiles(0, p->nx, p->ny, hype_strips(p->nx), c->nsy, tiles, c->ntiles) != c->ntiles)
+        rc = HYPE_E_INVALID;
+    else
+        rc = hyped_memcpy_h2d(c->tiles_d, tiles, (size_t)c->ntiles * sizeof(hype_tile), c->stream);
     if (rc == HYPE_OK) rc = hyped_stream_sync(c->stream);
     free(tiles);
     if (rc != HYPE_OK) goto fail;
@@ -345,26 +355,31 @@ int hype_dt_history(hype_ctx *c, double *dts, int n)
     return hyped_stream_sync(c->stream);
 }
 
-/* test hooks: tile height and persistent grid */
+/* tiling hooks: rows per tile (the grid is split into ceil(ny/hrows)
+ * near-equal segments) and the persistent grid size */
 int hype_ctx_set_tiling(hype_ctx *c, int hrows, int grid)
 {
     hype_tile *tiles;
-    int rc;
-    if (!c || hrows < 1) return HYPE_E_INVALID;
+    int rc, nsy;
+    if (!c || hrows < 1 || grid < 0) return HYPE_E_INVALID;
     if (c->graph) {
         hyped_graph_destroy(c->graph);
         c->graph = NULL;
     }
-    c->hrows = hrows;
+    nsy = (c->P.ny + hrows - 1) / hrows;
+    c->nsy = nsy;
     c->grid = grid;
-    c->ntiles = hype_count_tiles(c->P.nx, c->P.ny, hrows);
+    c->ntiles = hype_strips(c->P.nx) * nsy;
     hyped_free(c->tiles_d);
     c->tiles_d = NULL;
     HYPE_TRY(hyped_malloc((void **)&c->tiles_d, (size_t)c->ntiles * sizeof(hype_tile)));
     tiles = (hype_tile *)malloc((size_t)c->ntiles * sizeof(hype_tile));
     if (!tiles) return HYPE_E_NOMEM;
-    hype_make_tiles(0, c->P.nx, c->P.ny, hrows, tiles, c->ntiles);
-    c->inst_h.hrows = hrows;
+    if (hype_make_tiles(0, c->P.nx, c->P.ny, hype_strips(c->P.nx), nsy, tiles, c->ntiles) != c->ntiles) {
+        free(tiles);
+        return HYPE_E_INVALID;
+    }
+    c->inst_h.nsy = nsy;
     rc = hyped_memcpy_h2d(c->tiles_d, tiles, (size_t)c->ntiles * sizeof(hype_tile), c->stream);
     if (rc == HYPE_OK) rc = hyped_memcpy_h2d(c->inst_d, &c->inst_h, sizeof(hype_inst), c->stream);
     if (rc == HYPE_OK) rc = hyped_stream_sync(c->stream);
@@ -375,7 +390,7 @@ int hype_ctx_set_tiling(hype_ctx *c, int hrows, int grid)
 int hype_ctx_tiling(hype_ctx *c, int *hrows, int *ntiles, int *grid)
 {
     if (!c) return HYPE_E_INVALID;
-    if (hrows) *hrows = c->hrows;
+    if (hrows) *hrows = (c->P.ny + c->nsy - 1) / c->nsy;
     if (ntiles) *ntiles = c->ntiles;
     if (grid) *grid = c->grid;
     return HYPE_OK;

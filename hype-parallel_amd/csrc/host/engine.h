@@ -1,0 +1,71 @@
+/* engine.h -- multi-instance stepping engine shared by the batched and the
+ * served drivers (host side, C).
+ *
+ * Instances live in slots of a fixed-capacity table on the device.  Every
+ * kernel variant in use (system x path x order) has a group: a tile table on
+ * the device whose length is also read from device memory, a persistent-
+ * queue counter and a branch stream.  One time step of all instances is one
+ * persistent launch per RK stage per group; the launches of all groups are
+ * captured once into a hipGraph with one parallel branch per group, and
+ * replayed.  Admitting or retiring instances rewrites the tables between
+ * replays, never the graph.
+ */
+#ifndef HYPE_ENGINE_H
+#define HYPE_ENGINE_H
+
+#include "common.h"
+
+#define HYPE_MAX_GROUPS 18
+
+typedef struct {
+    hype_variant v;
+    int nstages;
+    hype_tile *tiles_d;     /* stepping table (capacity tile_cap) */
+    int *ntiles_d;          /* its device-resident length */
+    hype_tile *prime_d;     /* priming table of newly started instances */
+    int *nprime_d;
+    unsigned *queue_d;      /* {next, finished} for stepping, then priming */
+    int tile_cap;
+    int ntiles, nprime;     /* host mirrors */
+    int grid;               /* persistent workgroups */
+    void *stream;           /* graph branch */
+    void *ev_done;
+} eng_group;
+
+typedef struct {
+    int cap;                /* instance slots */
+    hype_inst *inst_h;
+    hype_inst *inst_d;
+    hype_dstate *st_d;      /* one per slot */
+    hype_problem *P;
+    int *used, *group_of, *nsy, *fresh;
+    int target_rows;        /* common tile height of all instances */
+    eng_group g[HYPE_MAX_GROUPS];
+    int ng;
+    void *stream;           /* main stream */
+    void *ev_fork;
+    void *graph;            /* one captured step */
+    int ncu;
+    unsigned flags;
+} hype_engine;
+
+int eng_init(hype_engine *E, int cap, int target_rows, unsigned flags);
+void eng_free(hype_engine *E);
+/* place an instance in a slot (its U, A, B device buffers) */
+int eng_attach(hype_engine *E, int slot, const hype_problem *p, double *U, double *A, double *B);
+int eng_detach(hype_engine *E, int slot);
+/* recompute the tile tables of every group from the used slots */
+int eng_commit(hype_engine *E);
+/* initialise the time state of the fresh slots and run their priming call */
+int eng_start(hype_engine *E, const int *slots, int n, const double *t0, const double *t_end,
+              const int *max_steps, int clip, double **hist, const int *hist_len);
+/* enqueue nsteps steps of every used slot (graph replays) */
+int eng_step(hype_engine *E, int nsteps);
+int eng_sync(hype_engine *E);
+int eng_read_states(hype_engine *E, hype_dstate *out);
+/* tile height the model picks for the used slots */
+int eng_pick_rows(hype_engine *E);
+int eng_set_rows(hype_engine *E, int rows);
+int eng_total_tiles(hype_engine *E);
+
+#endif

@@ -49,8 +49,7 @@ typedef struct hype_inst {
     hype_dstate *st;       /* device time state */
     int nx, ny;
     int bc[4];
-    int hrows;             /* rows per tile of this instance */
-    int ntiles_x;
+    int nsx, nsy;          /* strips x segments of the tiling */
     double h, r1_h, cfl;
     double x_min, y_min;
     double g1, g2, p1, p2;
@@ -58,10 +57,11 @@ typedef struct hype_inst {
     int pad1;
 } hype_inst;
 
-/* tile: instance, first owned column, first owned row, flags (bit 0 =
- * leader tile of its instance, updates the time state) */
+/* tile: instance, first owned column and row, owned width (<= HYPE_TILE_W)
+ * and height, flags (bit 0 = leader tile of its instance, which updates the
+ * time state) */
 typedef struct hype_tile {
-    int inst, x0, y0, flags;
+    int inst, x0, y0, w, h, flags;
 } hype_tile;
 
 /* packed error word: call sequence within the step (3 bits: 0 priming or
@@ -86,7 +86,9 @@ typedef struct hype_launch {
     hype_variant v;
     const hype_inst *insts;    /* device array */
     const hype_tile *tiles;    /* device array */
-    int ntiles;
+    int ntiles;                /* tile count, unless ntiles_dev is set */
+    const int *ntiles_dev;     /* device-resident tile count (tables that
+                                  change between graph replays) */
     int mode;
     int stage;
     int grid;                  /* persistent blocks; 0 = one per tile */
@@ -116,6 +118,8 @@ int hyped_cons2prim(int system, const double *U, double *W, long ncells,
 int hyped_copy(double *dst, const double *src, long n, void *stream);
 int hyped_mem_info(size_t *free_bytes, size_t *total_bytes);
 int hyped_num_cus(void);
+/* resident workgroups per CU of the stage kernel of a variant */
+int hyped_blocks_per_cu(const hype_variant *v);
 const char *hyped_last_error(void);
 
 /* HIP events on a stream */
@@ -123,6 +127,8 @@ int hyped_event_create(void **ev);
 int hyped_event_destroy(void *ev);
 int hyped_event_record(void *ev, void *stream);
 int hyped_event_elapsed_ms(void *a, void *b, float *ms);
+int hyped_event_create_sync(void **ev);   /* no timing: fork/join in graphs */
+int hyped_stream_wait(void *stream, void *ev);
 
 /* graph capture of a sequence of launches on one stream */
 int hyped_graph_begin(void *stream);

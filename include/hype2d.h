@@ -188,6 +188,44 @@ void hype_rankine_hugoniot(double gamma, double rho0, double p0, double M,
 int hype_ctx_set_tiling(hype_ctx *c, int hrows, int grid);
 int hype_ctx_tiling(hype_ctx *c, int *hrows, int *ntiles, int *grid);
 
+/* ======================================================================
+ * Batched mode: many independent instances (any mix of systems, orders,
+ * paths and grid sizes) advanced together.  The reference runs one grid
+ * per program (hype.c:14-456); each instance here follows exactly the
+ * single-instance solve of hype_begin/hype_advance (same dt sequence, same
+ * bits), with its own CFL dt, end time and step budget.  One step of the
+ * batch is one persistent launch per RK stage per kernel variant over a
+ * flattened tile queue of all instances, replayed from a captured hipGraph.
+ * ====================================================================== */
+typedef struct hype_batch hype_batch;
+
+typedef struct hype_batch_opts {
+    int tile_rows;         /* rows per tile; 0 = picked by the tile-shape model */
+} hype_batch_opts;
+
+int hype_batch_create(const hype_problem *probs, int n, const hype_batch_opts *opts,
+                      hype_batch **out);
+void hype_batch_destroy(hype_batch *b);
+int hype_batch_set_state(hype_batch *b, int i, const double *U, int mem);
+int hype_batch_get_state(hype_batch *b, int i, double *U, int mem);
+/* t0 may be NULL (all zero); flags: HYPE_TS_CLIP */
+int hype_batch_begin(hype_batch *b, const double *t0, const double *t_end,
+                     const int *max_steps, unsigned flags);
+/* enqueue nsteps steps of every instance; finished instances idle */
+int hype_batch_advance(hype_batch *b, int nsteps);
+int hype_batch_advance_timed(hype_batch *b, int nsteps, double *ms);
+/* status of every instance (st may be NULL); returns HYPE_E_INADMISSIBLE if
+ * any instance stopped on an inadmissible state (first such instance is
+ * described by hype_batch_last_error) */
+int hype_batch_sync(hype_batch *b, hype_status *st);
+const char *hype_batch_last_error(hype_batch *b, double *x, double *y);
+int hype_batch_tiling(hype_batch *b, int *rows, int *tiles, int *groups);
+int hype_batch_set_rows(hype_batch *b, int rows);
+/* Time `steps` steps at every candidate tile height and keep the fastest.
+ * Advances the current states: set them again afterwards. */
+int hype_batch_autotune(hype_batch *b, const int *cand, int ncand, int steps,
+                        int *best_rows, double *best_ms);
+
 #ifdef __cplusplus
 }
 #endif
