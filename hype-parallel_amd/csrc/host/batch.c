@@ -71,7 +71,8 @@ int hype_batch_create(const hype_problem *probs, int n, const hype_batch_opts *o
     }
     if ((rc = eng_init(&b->E, n, b->opts.tile_rows, 0)) != HYPE_OK ||
         (rc = hyped_malloc((void **)&b->slab, total * sizeof(double))) != HYPE_OK ||
-        (rc = hyped_event_create(&b->ev0)) != HYPE_OK || (rc = hyped_event_create(&b->ev1)) != HYPE_OK) {
+        (rc = hyped_event_create(&b->ev0)) != HYPE_OK || (rc = hyped_event_create(&b->ev1)) != HYPE_OK ||
+        (rc = hyped_fill_u64(b->slab, 0ull, total, b->E.stream)) != HYPE_OK) {
         hype_batch_destroy(b);
         return rc;
     }

@@ -4,7 +4,7 @@
 #include <stdlib.h>
 #include <string.h>
 
-#define GROUP_TILE_CAP (1 << 15)
+#define GROUP_TILE_CAP0 1024
 
 int eng_init(hype_engine *E, int cap, int target_rows, unsigned flags)
 {
@@ -74,7 +74,7 @@ static int find_group(hype_engine *E, const hype_variant *v)
         memset(G, 0, sizeof(*G));
         G->v = *v;
         G->nstages = hyped_num_stages(v->order);
-        G->tile_cap = GROUP_TILE_CAP;
+        G->tile_cap = GROUP_TILE_CAP0;
         if (hyped_malloc((void **)&G->tiles_d, (size_t)G->tile_cap * sizeof(hype_tile)) != HYPE_OK ||
             hyped_malloc((void **)&G->prime_d, (size_t)G->tile_cap * sizeof(hype_tile)) != HYPE_OK ||
             hyped_malloc((void **)&G->ntiles_d, 2 * sizeof(int)) != HYPE_OK ||
@@ -138,12 +138,38 @@ static int rows_for(const hype_engine *E, int slot)
     return nsy;
 }
 
-int eng_total_tiles(hype_engine *E)
+/* tiles of the used slots of group g (g < 0: all groups) */
+static int count_tiles(const hype_engine *E, int g, int only_fresh)
 {
     int s, t = 0;
     for (s = 0; s < E->cap; ++s)
-        if (E->used[s]) t += hype_strips(E->P[s].nx) * rows_for(E, s);
+        if (E->used[s] && (g < 0 || E->group_of[s] == g) && (!only_fresh || E->fresh[s]))
+            t += hype_strips(E->P[s].nx) * rows_for(E, s);
     return t;
+}
+
+int eng_total_tiles(hype_engine *E) { return count_tiles(E, -1, 0); }
+
+/* grow the tile tables of a group to hold n tiles; the stepping table is a
+ * kernel argument of the captured graph, so a new one invalidates it */
+static int ensure_cap(hype_engine *E, eng_group *G, int n)
+{
+    int cap = G->tile_cap;
+    if (n <= cap) return HYPE_OK;
+    while (cap < n) cap *= 2;
+    HYPE_TRY(hyped_stream_sync(E->stream));
+    hyped_free(G->tiles_d);
+    hyped_free(G->prime_d);
+    G->tiles_d = G->prime_d = NULL;
+    G->tile_cap = 0;
+    HYPE_TRY(hyped_malloc((void **)&G->tiles_d, (size_t)cap * sizeof(hype_tile)));
+    HYPE_TRY(hyped_malloc((void **)&G->prime_d, (size_t)cap * sizeof(hype_tile)));
+    G->tile_cap = cap;
+    if (E->graph) {
+        hyped_graph_destroy(E->graph);
+        E->graph = NULL;
+    }
+    return HYPE_OK;
 }
 
 /* Common tile height for the used slots: the tile-shape model of
@@ -211,14 +237,17 @@ static int build_tiles(hype_engine *E, int g, int only_fresh, hype_tile *buf, in
 
 int eng_commit(hype_engine *E)
 {
-    hype_tile *buf = (hype_tile *)malloc((size_t)GROUP_TILE_CAP * sizeof(hype_tile));
+    int total = count_tiles(E, -1, 0);
+    hype_tile *buf = (hype_tile *)malloc((size_t)(total > 0 ? total : 1) * sizeof(hype_tile));
     int g, rc = HYPE_OK;
     if (!buf) return HYPE_E_NOMEM;
     for (g = 0; g < E->ng && rc == HYPE_OK; ++g) {
         eng_group *G = &E->g[g];
-        int n = build_tiles(E, g, 0, buf, G->tile_cap);
+        int n;
+        if ((rc = ensure_cap(E, G, count_tiles(E, g, 0))) != HYPE_OK) break;
+        n = build_tiles(E, g, 0, buf, G->tile_cap);
         if (n < 0) {
-            rc = HYPE_E_UNSUPPORTED; /* more tiles than a group table holds */
+            rc = HYPE_E_INVALID; /* a grid the tiler cannot split */
             break;
         }
         G->ntiles = n;
@@ -258,7 +287,7 @@ int eng_start(hype_engine *E, const int *slots, int n, const double *t0, const d
               const int *max_steps, int clip, double **hist, const int *hist_len)
 {
     hype_tile *buf;
-    int k, g, rc = HYPE_OK;
+    int k, g, m, rc = HYPE_OK;
     for (k = 0; k < n; ++k) {
         hype_dstate d;
         int s = slots[k];
@@ -274,12 +303,14 @@ int eng_start(hype_engine *E, const int *slots, int n, const double *t0, const d
         HYPE_TRY(hyped_memcpy_h2d(E->st_d + s, &d, sizeof(d), E->stream));
         E->fresh[s] = 2; /* marked for priming */
     }
-    buf = (hype_tile *)malloc((size_t)GROUP_TILE_CAP * sizeof(hype_tile));
-    if (!buf) return HYPE_E_NOMEM;
     for (k = 0; k < E->cap; ++k) E->fresh[k] = (E->fresh[k] == 2);
+    m = count_tiles(E, -1, 1);
+    buf = (hype_tile *)malloc((size_t)(m > 0 ? m : 1) * sizeof(hype_tile));
+    if (!buf) return HYPE_E_NOMEM;
     for (g = 0; g < E->ng && rc == HYPE_OK; ++g) {
         eng_group *G = &E->g[g];
-        int m = build_tiles(E, g, 1, buf, G->tile_cap);
+        if ((rc = ensure_cap(E, G, count_tiles(E, g, 1))) != HYPE_OK) break;
+        m = build_tiles(E, g, 1, buf, G->tile_cap);
         if (m < 0) {
             rc = HYPE_E_UNSUPPORTED;
             break;

@@ -1,0 +1,413 @@
+/*
+ * serve.c -- request loop with continuous batching (host side, C).
+ *
+ * The reference solves one configured problem per program run
+ * (hype.c:14-456).  The server accepts a stream of independent solve
+ * requests, each with its own system, grid, end time and step budget, and
+ * keeps as many of them resident in HBM as a byte budget allows.  Every
+ * round (hype_server_poll) it
+ *   1. reads the time state of the running instances (one small copy),
+ *   2. retires the finished ones: result to the caller's buffer, slot and
+ *      HBM block freed,
+ *   3. admits queued requests in arrival order while slots and budget allow,
+ *   4. rebuilds the tile tables and primes the new instances, and
+ *   5. enqueues the next steps of everything running: one persistent launch
+ *      per RK stage per kernel variant, replayed from a captured graph.
+ * An instance's arithmetic does not depend on what else runs: results are
+ * bitwise those of a solo hype_ctx solve.
+ *
+ * Device states live in one arena allocated at creation; blocks are placed
+ * first-fit and coalesced on release, so admission never calls the device
+ * allocator.
+ */
+#define _GNU_SOURCE
+#include <stdlib.h>
+#include <string.h>
+#include <time.h>
+
+#include "engine.h"
+
+#define ARENA_ALIGN 256
+
+typedef struct {
+    size_t off, size;
+} arena_blk;
+
+typedef struct {
+    char *base;
+    size_t size, used;
+    arena_blk *free_;       /* sorted by offset, coalesced */
+    int nfree, capfree;
+} arena;
+
+typedef struct {
+    hype_request r;
+    int state;
+    int slot;
+    size_t off, bytes;
+    double t_submit, t_start, t_done;
+    hype_status st;
+} srv_req;
+
+struct hype_server {
+    hype_server_opts o;
+    hype_engine E;
+    arena A;
+    srv_req *req;
+    int nreq, capreq;
+    int next;               /* first request not yet admitted (FIFO) */
+    int *slot_req;          /* request running in each slot, or -1 */
+    int active;
+    int steps;              /* steps enqueued by the last round */
+    double ms_per_step;     /* measured GPU time of one step of the mix */
+    void *ev0, *ev1;
+    int timing;             /* ev0/ev1 bracket the last round's steps */
+    hype_dstate *d;
+    struct timespec t0;
+};
+
+/* ---- arena ------------------------------------------------------------ */
+static size_t round_up(size_t n) { return (n + ARENA_ALIGN - 1) / ARENA_ALIGN * ARENA_ALIGN; }
+
+static int arena_init(arena *A, size_t bytes)
+{
+    memset(A, 0, sizeof(*A));
+    A->size = bytes / ARENA_ALIGN * ARENA_ALIGN;
+    if (A->size == 0) return HYPE_E_INVALID;
+    HYPE_TRY(hyped_malloc((void **)&A->base, A->size));
+    A->capfree = 64;
+    A->free_ = (arena_blk *)malloc((size_t)A->capfree * sizeof(arena_blk));
+    if (!A->free_) return HYPE_E_NOMEM;
+    A->free_[0].off = 0;
+    A->free_[0].size = A->size;
+    A->nfree = 1;
+    return HYPE_OK;
+}
+
+static void arena_free_all(arena *A)
+{
+    hyped_free(A->base);
+    free(A->free_);
+    memset(A, 0, sizeof(*A));
+}
+
+/* first fit; returns 0 and the offset, or -1 if no block is large enough */
+static int arena_alloc(arena *A, size_t bytes, size_t *off)
+{
+    int k;
+    bytes = round_up(bytes);
+    for (k = 0; k < A->nfree; ++k)
+        if (A->free_[k].size >= bytes) {
+            *off = A->free_[k].off;
+            A->free_[k].off += bytes;
+            A->free_[k].size -= bytes;
+            if (A->free_[k].size == 0) {
+                memmove(A->free_ + k, A->free_ + k + 1, (size_t)(A->nfree - k - 1) * sizeof(arena_blk));
+                --A->nfree;
+            }
+            A->used += bytes;
+            return 0;
+        }
+    return -1;
+}
+
+static int arena_release(arena *A, size_t off, size_t bytes)
+{
+    int k = 0;
+    bytes = round_up(bytes);
+    while (k < A->nfree && A->free_[k].off < off) ++k;
+    if (A->nfree == A->capfree) {
+        arena_blk *nb = (arena_blk *)realloc(A->free_, (size_t)A->capfree * 2 * sizeof(arena_blk));
+        if (!nb) return HYPE_E_NOMEM;
+        A->free_ = nb;
+        A->capfree *= 2;
+    }
+    memmove(A->free_ + k + 1, A->free_ + k, (size_t)(A->nfree - k) * sizeof(arena_blk));
+    A->free_[k].off = off;
+    A->free_[k].size = bytes;
+    ++A->nfree;
+    A->used -= bytes;
+    if (k + 1 < A->nfree && A->free_[k].off + A->free_[k].size == A->free_[k + 1].off) {
+        A->free_[k].size += A->free_[k + 1].size;
+        memmove(A->free_ + k + 1, A->free_ + k + 2, (size_t)(A->nfree - k - 2) * sizeof(arena_blk));
+        --A->nfree;
+    }
+    if (k > 0 && A->free_[k - 1].off + A->free_[k - 1].size == A->free_[k].off) {
+        A->free_[k - 1].size += A->free_[k].size;
+        memmove(A->free_ + k, A->free_ + k + 1, (size_t)(A->nfree - k - 1) * sizeof(arena_blk));
+        --A->nfree;
+    }
+    return HYPE_OK;
+}
+
+/* ---- server ----------------------------------------------------------- */
+static double now_s(const hype_server *s)
+{
+    struct timespec t;
+    clock_gettime(CLOCK_MONOTONIC, &t);
+    return (double)(t.tv_sec - s->t0.tv_sec) + 1e-9 * (double)(t.tv_nsec - s->t0.tv_nsec);
+}
+
+static size_t state_bytes(const hype_problem *p)
+{
+    return (size_t)p->nx * p->ny * hype_nvar(p->system) * sizeof(double);
+}
+
+void hype_server_destroy(hype_server *s)
+{
+    if (!s) return;
+    eng_free(&s->E);
+    arena_free_all(&s->A);
+    if (s->ev0) hyped_event_destroy(s->ev0);
+    if (s->ev1) hyped_event_destroy(s->ev1);
+    free(s->req);
+    free(s->slot_req);
+    free(s->d);
+    free(s);
+}
+
+int hype_server_create(const hype_server_opts *opts, hype_server **out)
+{
+    hype_server *s;
+    size_t budget;
+    int k, rc;
+    if (!out) return HYPE_E_INVALID;
+    *out = NULL;
+    s = (hype_server *)calloc(1, sizeof(*s));
+    if (!s) return HYPE_E_NOMEM;
+    if (opts) s->o = *opts;
+    if (s->o.max_active < 1) s->o.max_active = 512;
+    if (s->o.round_ms <= 0.0) s->o.round_ms = 20.0;
+    clock_gettime(CLOCK_MONOTONIC, &s->t0);
+    budget = s->o.hbm_budget_bytes;
+    if ((rc = hyped_init()) != HYPE_OK) {
+        free(s);
+        return rc;
+    }
+    if (budget == 0) {
+        size_t fr = 0, tot = 0;
+        if ((rc = hyped_mem_info(&fr, &tot)) != HYPE_OK) {
+            free(s);
+            return rc;
+        }
+        budget = fr / 4 * 3;
+    }
+    s->o.hbm_budget_bytes = budget;
+    s->slot_req = (int *)malloc((size_t)s->o.max_active * sizeof(int));
+    s->d = (hype_dstate *)malloc((size_t)s->o.max_active * sizeof(hype_dstate));
+    if (!s->slot_req || !s->d) {
+        hype_server_destroy(s);
+        return HYPE_E_NOMEM;
+    }
+    for (k = 0; k < s->o.max_active; ++k) s->slot_req[k] = -1;
+    if ((rc = eng_init(&s->E, s->o.max_active, s->o.tile_rows, 0)) != HYPE_OK ||
+        (rc = arena_init(&s->A, budget)) != HYPE_OK || (rc = hyped_event_create(&s->ev0)) != HYPE_OK ||
+        (rc = hyped_event_create(&s->ev1)) != HYPE_OK) {
+        hype_server_destroy(s);
+        return rc;
+    }
+    *out = s;
+    return HYPE_OK;
+}
+
+double hype_server_now(hype_server *s) { return s ? now_s(s) : 0.0; }
+
+size_t hype_server_bytes_in_use(hype_server *s) { return s ? s->A.used : 0; }
+
+int hype_server_submit(hype_server *s, const hype_request *r, int *id)
+{
+    int nv;
+    srv_req *q;
+    if (!s || !r || !r->U0) return HYPE_E_INVALID;
+    HYPE_TRY(hype_check_problem(&r->problem, &nv));
+    if (r->problem.order < 1) return HYPE_E_UNSUPPORTED;
+    if (round_up(3 * state_bytes(&r->problem)) > s->A.size) return HYPE_E_NOMEM; /* never admissible */
+    if (s->nreq == s->capreq) {
+        int cap = s->capreq ? 2 * s->capreq : 256;
+        srv_req *nr = (srv_req *)realloc(s->req, (size_t)cap * sizeof(srv_req));
+        if (!nr) return HYPE_E_NOMEM;
+        s->req = nr;
+        s->capreq = cap;
+    }
+    q = &s->req[s->nreq];
+    memset(q, 0, sizeof(*q));
+    q->r = *r;
+    q->state = HYPE_REQ_QUEUED;
+    q->slot = -1;
+    q->bytes = 3 * state_bytes(&r->problem);
+    q->t_submit = now_s(s);
+    if (id) *id = s->nreq;
+    ++s->nreq;
+    return HYPE_OK;
+}
+
+static int finished(const hype_dstate *d, unsigned long long *err)
+{
+    unsigned long long e = d->step > 0 ? d->err[(d->step - 1) & 1] : d->err[1];
+    *err = e;
+    return d->done || d->step >= d->max_steps || !(d->t < d->t_end) || e != ~0ull;
+}
+
+static int retire(hype_server *s, int slot, double t)
+{
+    srv_req *q = &s->req[s->slot_req[slot]];
+    const hype_dstate *d = &s->d[slot];
+    unsigned long long e;
+    finished(d, &e);
+    if (q->r.U_out) {
+        HYPE_TRY(hyped_memcpy_d2h(q->r.U_out, s->A.base + q->off, state_bytes(&q->r.problem), s->E.stream));
+        HYPE_TRY(hyped_stream_sync(s->E.stream));
+    }
+    q->st.t = d->t;
+    q->st.dt = d->dt;
+    q->st.steps = d->step;
+    q->st.done = 1;
+    q->st.err = e != ~0ull ? HYPE_E_INADMISSIBLE : HYPE_OK;
+    q->t_done = t;
+    q->state = HYPE_REQ_DONE;
+    q->slot = -1;
+    HYPE_TRY(eng_detach(&s->E, slot));
+    HYPE_TRY(arena_release(&s->A, q->off, q->bytes));
+    s->slot_req[slot] = -1;
+    --s->active;
+    return HYPE_OK;
+}
+
+static int start_fresh(hype_server *s, const int *slots, int n, int clip)
+{
+    double *t0, *t_end;
+    int *maxs, k, m = 0, rc;
+    if (n == 0) return HYPE_OK;
+    t0 = (double *)malloc((size_t)n * sizeof(double));
+    t_end = (double *)malloc((size_t)n * sizeof(double));
+    maxs = (int *)malloc((size_t)n * sizeof(int));
+    int *sl = (int *)malloc((size_t)n * sizeof(int));
+    if (!t0 || !t_end || !maxs || !sl) {
+        free(t0);
+        free(t_end);
+        free(maxs);
+        free(sl);
+        return HYPE_E_NOMEM;
+    }
+    for (k = 0; k < n; ++k) {
+        const srv_req *q = &s->req[s->slot_req[slots[k]]];
+        if (((q->r.flags & HYPE_TS_CLIP) != 0) != (clip != 0)) continue;
+        sl[m] = slots[k];
+        t0[m] = q->r.t0;
+        t_end[m] = q->r.t_end;
+        maxs[m] = q->r.max_steps;
+        ++m;
+    }
+    rc = m ? eng_start(&s->E, sl, m, t0, t_end, maxs, clip, NULL, NULL) : HYPE_OK;
+    free(t0);
+    free(t_end);
+    free(maxs);
+    free(sl);
+    return rc;
+}
+
+int hype_server_poll(hype_server *s, int *active, int *queued)
+{
+    int slot, changed = 0, nfresh = 0, rc = HYPE_OK;
+    int *fresh;
+    double t;
+    if (!s) return HYPE_E_INVALID;
+    fresh = (int *)malloc((size_t)s->o.max_active * sizeof(int));
+    if (!fresh) return HYPE_E_NOMEM;
+    /* 1-2: retire what finished during the last round's steps */
+    if (s->active > 0) {
+        if ((rc = eng_read_states(&s->E, s->d)) != HYPE_OK) goto out;
+        if (s->timing && s->steps > 0) {
+            float ms = 0.0f;
+            if (hyped_event_elapsed_ms(s->ev0, s->ev1, &ms) == HYPE_OK && ms > 0.0f)
+                s->ms_per_step = (double)ms / s->steps;
+        }
+        t = now_s(s);
+        for (slot = 0; slot < s->o.max_active; ++slot) {
+            unsigned long long e;
+            if (s->slot_req[slot] < 0 || !finished(&s->d[slot], &e)) continue;
+            if ((rc = retire(s, slot, t)) != HYPE_OK) goto out;
+            changed = 1;
+        }
+    }
+    /* 3: admit in arrival order while a slot and HBM are free */
+    t = now_s(s);
+    while (s->next < s->nreq) {
+        srv_req *q = &s->req[s->next];
+        double *U;
+        size_t m;
+        for (slot = 0; slot < s->o.max_active && s->slot_req[slot] >= 0; ++slot) {
+        }
+        if (slot == s->o.max_active) break;
+        if (arena_alloc(&s->A, q->bytes, &q->off) != 0) break;
+        U = (double *)(s->A.base + q->off);
+        m = (size_t)q->r.problem.nx * q->r.problem.ny * hype_nvar(q->r.problem.system);
+        if ((rc = hyped_memcpy_h2d(U, q->r.U0, m * sizeof(double), s->E.stream)) != HYPE_OK ||
+            (rc = eng_attach(&s->E, slot, &q->r.problem, U, U + m, U + 2 * m)) != HYPE_OK) {
+            arena_release(&s->A, q->off, q->bytes);
+            goto out;
+        }
+        q->slot = slot;
+        q->state = HYPE_REQ_RUNNING;
+        q->t_start = t;
+        s->slot_req[slot] = s->next;
+        fresh[nfresh++] = slot;
+        ++s->active;
+        ++s->next;
+        changed = 1;
+    }
+    /* 4: new tile tables, priming of the admitted instances */
+    if (changed && s->active > 0) {
+        if (s->o.tile_rows < 1) eng_set_rows(&s->E, eng_pick_rows(&s->E));
+        if ((rc = eng_commit(&s->E)) != HYPE_OK) goto out;
+        if ((rc = start_fresh(s, fresh, nfresh, 1)) != HYPE_OK) goto out;
+        if ((rc = start_fresh(s, fresh, nfresh, 0)) != HYPE_OK) goto out;
+    }
+    /* 5: the next steps of everything running */
+    s->steps = 0;
+    s->timing = 0;
+    if (s->active > 0) {
+        int n = s->o.steps_per_round;
+        if (n < 1) {
+            n = s->ms_per_step > 0.0 ? (int)(s->o.round_ms / s->ms_per_step) : 4;
+            if (n < 1) n = 1;
+            if (n > 256) n = 256;
+        }
+        if ((rc = hyped_event_record(s->ev0, s->E.stream)) != HYPE_OK) goto out;
+        if ((rc = eng_step(&s->E, n)) != HYPE_OK) goto out;
+        if ((rc = hyped_event_record(s->ev1, s->E.stream)) != HYPE_OK) goto out;
+        s->steps = n;
+        s->timing = 1;
+    }
+out:
+    free(fresh);
+    if (active) *active = s->active;
+    if (queued) *queued = s->nreq - s->next;
+    return rc;
+}
+
+int hype_server_drain(hype_server *s)
+{
+    int a = 0, q = 0;
+    if (!s) return HYPE_E_INVALID;
+    do {
+        HYPE_TRY(hype_server_poll(s, &a, &q));
+    } while (a > 0 || q > 0);
+    return HYPE_OK;
+}
+
+int hype_server_info(hype_server *s, int id, hype_request_info *info)
+{
+    const srv_req *q;
+    if (!s || !info) return HYPE_E_INVALID;
+    if (id < 0 || id >= s->nreq) return HYPE_E_NOTFOUND;
+    q = &s->req[id];
+    memset(info, 0, sizeof(*info));
+    info->state = q->state;
+    info->status = q->st;
+    info->t_submit = q->t_submit;
+    info->t_start = q->t_start;
+    info->t_done = q->t_done;
+    info->bytes = q->bytes;
+    return HYPE_OK;
+}

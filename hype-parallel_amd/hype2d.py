@@ -59,7 +59,7 @@ class Status(C.Structure):
 
 
 class BatchOpts(C.Structure):
-    _fields_ = [("tile_rows", C.c_int), ("use_graph", C.c_int), ("persistent_blocks", C.c_int)]
+    _fields_ = [("tile_rows", C.c_int)]
 
 
 class ServerOpts(C.Structure):
@@ -68,6 +68,7 @@ class ServerOpts(C.Structure):
         ("max_active", C.c_int),
         ("steps_per_round", C.c_int),
         ("tile_rows", C.c_int),
+        ("round_ms", C.c_double),
     ]
 
 
@@ -75,8 +76,25 @@ class Request(C.Structure):
     _fields_ = [
         ("problem", Problem),
         ("U0", C.POINTER(C.c_double)),
+        ("U_out", C.POINTER(C.c_double)),
+        ("t0", C.c_double),
         ("t_end", C.c_double),
         ("max_steps", C.c_int),
+        ("flags", C.c_uint),
+    ]
+
+
+REQ_QUEUED, REQ_RUNNING, REQ_DONE = 0, 1, 2
+
+
+class RequestInfo(C.Structure):
+    _fields_ = [
+        ("status", Status),
+        ("state", C.c_int),
+        ("t_submit", C.c_double),
+        ("t_start", C.c_double),
+        ("t_done", C.c_double),
+        ("bytes", C.c_size_t),
     ]
 
 
@@ -131,6 +149,32 @@ def lib():
     L.hype_rankine_hugoniot.restype = None
     L.hype_ctx_set_tiling.argtypes = [vp, C.c_int, C.c_int]
     L.hype_ctx_tiling.argtypes = [vp, P(C.c_int), P(C.c_int), P(C.c_int)]
+    ip = P(C.c_int)
+    L.hype_batch_create.argtypes = [P(Problem), C.c_int, P(BatchOpts), P(vp)]
+    L.hype_batch_destroy.argtypes = [vp]
+    L.hype_batch_destroy.restype = None
+    L.hype_batch_set_state.argtypes = [vp, C.c_int, dp, C.c_int]
+    L.hype_batch_get_state.argtypes = [vp, C.c_int, dp, C.c_int]
+    L.hype_batch_begin.argtypes = [vp, dp, dp, ip, C.c_uint]
+    L.hype_batch_advance.argtypes = [vp, C.c_int]
+    L.hype_batch_advance_timed.argtypes = [vp, C.c_int, dp]
+    L.hype_batch_sync.argtypes = [vp, P(Status)]
+    L.hype_batch_last_error.argtypes = [vp, dp, dp]
+    L.hype_batch_last_error.restype = C.c_char_p
+    L.hype_batch_tiling.argtypes = [vp, ip, ip, ip]
+    L.hype_batch_set_rows.argtypes = [vp, C.c_int]
+    L.hype_batch_autotune.argtypes = [vp, ip, C.c_int, C.c_int, ip, dp]
+    L.hype_server_create.argtypes = [P(ServerOpts), P(vp)]
+    L.hype_server_destroy.argtypes = [vp]
+    L.hype_server_destroy.restype = None
+    L.hype_server_submit.argtypes = [vp, P(Request), ip]
+    L.hype_server_poll.argtypes = [vp, ip, ip]
+    L.hype_server_drain.argtypes = [vp]
+    L.hype_server_info.argtypes = [vp, C.c_int, P(RequestInfo)]
+    L.hype_server_now.argtypes = [vp]
+    L.hype_server_now.restype = C.c_double
+    L.hype_server_bytes_in_use.argtypes = [vp]
+    L.hype_server_bytes_in_use.restype = C.c_size_t
     _lib = L
     return L
 
@@ -142,7 +186,12 @@ def exported_symbols():
         "hype_cons2prim", "hype_set_state", "hype_get_state", "hype_begin", "hype_advance",
         "hype_sync", "hype_dt_history", "hype_last_error", "hype_stream",
         "hype_ctx_set_tiling", "hype_ctx_tiling", "hype_advance_timed", "hype_initial_state",
-        "hype_rankine_hugoniot",
+        "hype_rankine_hugoniot", "hype_batch_create", "hype_batch_destroy", "hype_batch_set_state",
+        "hype_batch_get_state", "hype_batch_begin", "hype_batch_advance", "hype_batch_advance_timed",
+        "hype_batch_sync", "hype_batch_last_error", "hype_batch_tiling", "hype_batch_set_rows",
+        "hype_batch_autotune", "hype_server_create", "hype_server_destroy", "hype_server_submit",
+        "hype_server_poll", "hype_server_drain", "hype_server_info", "hype_server_now",
+        "hype_server_bytes_in_use",
     ]
 
 
@@ -293,3 +342,179 @@ class Context:
         self.advance(steps)
         st = self.sync()
         return self.get_state(), st
+
+
+class Batch:
+    """Many independent instances advanced together (hype_batch)."""
+
+    def __init__(self, problems, tile_rows=0):
+        self.L = lib()
+        self.problems = list(problems)
+        n = len(self.problems)
+        arr = (Problem * n)(*self.problems)
+        opts = BatchOpts(tile_rows)
+        h = C.c_void_p()
+        rc = self.L.hype_batch_create(arr, n, C.byref(opts), C.byref(h))
+        if rc != OK:
+            raise HypeError(rc, "hype_batch_create")
+        self.h = h
+        self.shapes = [(p.ny, p.nx, NVAR[p.system]) for p in self.problems]
+
+    def __len__(self):
+        return len(self.problems)
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.hype_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _check(self, rc, what, allow_inadmissible=False):
+        if rc == OK or (allow_inadmissible and rc == E_INADMISSIBLE):
+            return rc
+        msg = what
+        if rc == E_INADMISSIBLE:
+            msg += ": " + self.last_error()[0]
+        raise HypeError(rc, msg)
+
+    def last_error(self):
+        x, y = C.c_double(), C.c_double()
+        s = self.L.hype_batch_last_error(self.h, C.byref(x), C.byref(y))
+        return (s.decode() if s else ""), x.value, y.value
+
+    def set_state(self, i, U):
+        U = np.ascontiguousarray(U, dtype=np.float64).reshape(self.shapes[i])
+        self._check(self.L.hype_batch_set_state(self.h, i, _dptr(U), MEM_HOST), "hype_batch_set_state")
+
+    def get_state(self, i):
+        U = np.empty(self.shapes[i], dtype=np.float64)
+        self._check(self.L.hype_batch_get_state(self.h, i, _dptr(U), MEM_HOST), "hype_batch_get_state")
+        return U
+
+    def begin(self, t_end=None, max_steps=None, t0=None, flags=TS_CLIP):
+        n = len(self)
+        te = np.full(n, np.inf) if t_end is None else np.ascontiguousarray(np.broadcast_to(t_end, (n,)), dtype=np.float64)
+        ms = np.full(n, 1 << 30, dtype=np.int32) if max_steps is None else \
+            np.ascontiguousarray(np.broadcast_to(max_steps, (n,)), dtype=np.int32)
+        t0p = None
+        if t0 is not None:
+            t0a = np.ascontiguousarray(np.broadcast_to(t0, (n,)), dtype=np.float64)
+            t0p = _dptr(t0a)
+        self._check(self.L.hype_batch_begin(self.h, t0p, _dptr(te), ms.ctypes.data_as(C.POINTER(C.c_int)), flags),
+                    "hype_batch_begin")
+
+    def advance(self, nsteps):
+        self._check(self.L.hype_batch_advance(self.h, nsteps), "hype_batch_advance")
+
+    def advance_timed(self, nsteps):
+        ms = C.c_double()
+        self._check(self.L.hype_batch_advance_timed(self.h, nsteps, C.byref(ms)), "hype_batch_advance_timed")
+        return ms.value
+
+    def sync(self, allow_inadmissible=False):
+        st = (Status * len(self))()
+        rc = self.L.hype_batch_sync(self.h, st)
+        self._check(rc, "hype_batch_sync", allow_inadmissible)
+        return list(st)
+
+    def tiling(self):
+        a, b, c = C.c_int(), C.c_int(), C.c_int()
+        self._check(self.L.hype_batch_tiling(self.h, C.byref(a), C.byref(b), C.byref(c)), "hype_batch_tiling")
+        return a.value, b.value, c.value
+
+    def set_rows(self, rows):
+        self._check(self.L.hype_batch_set_rows(self.h, rows), "hype_batch_set_rows")
+
+    def autotune(self, candidates, steps=4):
+        c = np.ascontiguousarray(candidates, dtype=np.int32)
+        r, ms = C.c_int(), C.c_double()
+        self._check(self.L.hype_batch_autotune(self.h, c.ctypes.data_as(C.POINTER(C.c_int)), len(c), steps,
+                                               C.byref(r), C.byref(ms)), "hype_batch_autotune")
+        return r.value, ms.value
+
+
+class Server:
+    """Request loop with continuous batching (hype_server)."""
+
+    def __init__(self, hbm_budget_bytes=0, max_active=0, steps_per_round=0, tile_rows=0, round_ms=0.0):
+        self.L = lib()
+        o = ServerOpts(hbm_budget_bytes, max_active, steps_per_round, tile_rows, round_ms)
+        h = C.c_void_p()
+        rc = self.L.hype_server_create(C.byref(o), C.byref(h))
+        if rc != OK:
+            raise HypeError(rc, "hype_server_create")
+        self.h = h
+        self._keep = {}  # request id -> (U0, U_out) kept alive until done
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.hype_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def submit(self, problem, U0, t_end=float("inf"), max_steps=1 << 30, t0=0.0, flags=TS_CLIP):
+        shape = (problem.ny, problem.nx, NVAR[problem.system])
+        U0 = np.ascontiguousarray(U0, dtype=np.float64).reshape(shape)
+        out = np.empty(shape, dtype=np.float64)
+        r = Request(problem, _dptr(U0), _dptr(out), t0, t_end, max_steps, flags)
+        rid = C.c_int()
+        rc = self.L.hype_server_submit(self.h, C.byref(r), C.byref(rid))
+        if rc != OK:
+            raise HypeError(rc, "hype_server_submit")
+        self._keep[rid.value] = (U0, out)
+        return rid.value
+
+    def poll(self):
+        a, q = C.c_int(), C.c_int()
+        rc = self.L.hype_server_poll(self.h, C.byref(a), C.byref(q))
+        if rc != OK:
+            raise HypeError(rc, "hype_server_poll")
+        return a.value, q.value
+
+    def drain(self):
+        rc = self.L.hype_server_drain(self.h)
+        if rc != OK:
+            raise HypeError(rc, "hype_server_drain")
+
+    def info(self, rid):
+        inf = RequestInfo()
+        rc = self.L.hype_server_info(self.h, rid, C.byref(inf))
+        if rc != OK:
+            raise HypeError(rc, "hype_server_info")
+        return inf
+
+    def result(self, rid):
+        """(U, info) of a finished request; the result buffer is released."""
+        inf = self.info(rid)
+        if inf.state != REQ_DONE:
+            raise HypeError(E_BUSY, f"request {rid} not finished")
+        U0, out = self._keep.pop(rid)
+        return out, inf
+
+    def now(self):
+        return self.L.hype_server_now(self.h)
+
+    def bytes_in_use(self):
+        return self.L.hype_server_bytes_in_use(self.h)

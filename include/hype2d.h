@@ -226,6 +226,60 @@ int hype_batch_set_rows(hype_batch *b, int rows);
 int hype_batch_autotune(hype_batch *b, const int *cand, int ncand, int steps,
                         int *best_rows, double *best_ms);
 
+/* ======================================================================
+ * Served mode: a stream of independent solve requests with continuous
+ * batching.  Requests are admitted in arrival order while instance slots
+ * and the HBM budget allow; each round retires finished instances, admits
+ * queued ones and advances everything running by a few steps.  A request's
+ * result is bitwise the result of the same solve run alone.  The reference
+ * has no counterpart (one problem per program run, hype.c:14-456).
+ * ====================================================================== */
+typedef struct hype_server hype_server;
+
+typedef struct hype_server_opts {
+    size_t hbm_budget_bytes; /* device bytes for instance states (3 per
+                                instance: U and two stage buffers);
+                                0 = three quarters of the free memory */
+    int max_active;          /* instance slots; 0 = 512 */
+    int steps_per_round;     /* 0 = as many as fit round_ms */
+    int tile_rows;           /* 0 = tile-shape model at every admission */
+    double round_ms;         /* target GPU time of a round; 0 = 20 ms */
+} hype_server_opts;
+
+typedef struct hype_request {
+    hype_problem problem;    /* order 1 or 2 */
+    const double *U0;        /* host initial state, read at admission */
+    double *U_out;           /* host result buffer (may be NULL), written at
+                                completion; both must stay valid until then */
+    double t0, t_end;
+    int max_steps;
+    unsigned flags;          /* HYPE_TS_CLIP */
+} hype_request;
+
+#define HYPE_REQ_QUEUED 0
+#define HYPE_REQ_RUNNING 1
+#define HYPE_REQ_DONE 2
+
+typedef struct hype_request_info {
+    hype_status status;      /* final status once done */
+    int state;               /* HYPE_REQ_* */
+    double t_submit, t_start, t_done; /* server clock (hype_server_now), s */
+    size_t bytes;            /* device bytes held while running */
+} hype_request_info;
+
+int hype_server_create(const hype_server_opts *opts, hype_server **out);
+void hype_server_destroy(hype_server *s);
+/* queue a request (HYPE_E_NOMEM if it can never fit the budget) */
+int hype_server_submit(hype_server *s, const hype_request *r, int *id);
+/* one round: retire, admit, enqueue steps; returns the running and queued
+ * counts.  Blocks until the previous round's steps have finished. */
+int hype_server_poll(hype_server *s, int *active, int *queued);
+/* poll until every submitted request is done */
+int hype_server_drain(hype_server *s);
+int hype_server_info(hype_server *s, int id, hype_request_info *info);
+double hype_server_now(hype_server *s);
+size_t hype_server_bytes_in_use(hype_server *s);
+
 #ifdef __cplusplus
 }
 #endif
