@@ -9,6 +9,8 @@ HBM roofline of the stage kernel and a CPU baseline (the FP64 C oracle on the
 host cores, bounded sample).
 
     python bench.py [--steps K] [--warmup W] [--config kapila|euler|bn]
+    python bench.py --config batched    # 64 uneven Kapila instances at once
+    python bench.py --config served     # Poisson request stream, continuous batching
 """
 import argparse
 import json
@@ -57,6 +59,38 @@ def workload(name):
     raise SystemExit(f"unknown config {name}")
 
 
+def dt_estimate(p, U):
+    """Initial-dt estimate CFL*h/(2*max(|u|+c)) from cell averages (frozen
+    sound speeds); used only to turn a requested step count into an end time."""
+    g1, g2, p1, p2 = p.gamma1, p.gamma2, p.pi1, p.pi2
+    if p.system == H.EULER:
+        rho = U[..., 0]
+        u, v = U[..., 1] / rho, U[..., 2] / rho
+        pr = (g1 - 1.0) * (U[..., 3] - 0.5 * rho * (u * u + v * v))
+        c = np.sqrt(np.maximum(g1 * pr / rho, 0.0))
+        s = np.maximum(np.abs(u), np.abs(v)) + c
+    elif p.system == H.KAPILA:
+        rho = U[..., 0] + U[..., 1]
+        u, v, a1 = U[..., 2] / rho, U[..., 3] / rho, U[..., 5]
+        a2 = 1.0 - a1
+        e = U[..., 4] - 0.5 * rho * (u * u + v * v)
+        pr = (e - a1 * g1 * p1 / (g1 - 1.0) - a2 * g2 * p2 / (g2 - 1.0)) / (a1 / (g1 - 1.0) + a2 / (g2 - 1.0))
+        c2 = (U[..., 0] * g1 * (pr + p1) / np.maximum(U[..., 0] / a1, 1e-300)
+              + U[..., 1] * g2 * (pr + p2) / np.maximum(U[..., 1] / np.maximum(a2, 1e-300), 1e-300)) / rho
+        s = np.maximum(np.abs(u), np.abs(v)) + np.sqrt(np.maximum(c2, 0.0))
+    else:
+        phi = U[..., 8]
+        s = 0.0
+        for q, a, g, pi in ((U[..., 0:4], phi, g1, p1), (U[..., 4:8], 1.0 - phi, g2, p2)):
+            r = q[..., 0] / a
+            u, v = q[..., 1] / q[..., 0], q[..., 2] / q[..., 0]
+            pr = (q[..., 3] / q[..., 0] - 0.5 * (u * u + v * v)) * r * (g - 1.0) - g * pi
+            c = np.sqrt(np.maximum(g * (pr + pi) / r, 0.0))
+            s = np.maximum(s, np.maximum(np.abs(u), np.abs(v)) + c)
+    h = (p.x_max - p.x_min) / p.nx
+    return p.cfl * h / (2.0 * float(np.max(s)))
+
+
 def bytes_per_cell_step(nv, order):
     # fused stage kernels: read the stage input, write the stage output,
     # plus one extra read of U^n in the combination stage of SSPRK(4,3)
@@ -86,6 +120,209 @@ def cpu_baseline(p, U0, nv, budget_s=12.0):
             "sample": f"{k} SSP-RK steps of the same {p.nx}x{p.ny} workload from its initial state "
                       f"(C oracle, OpenMP, {threads} threads; priming and allocation cancelled by "
                       f"differencing a 1-step and a {k + 1}-step run)"}
+
+
+def load_traffic(workload_name):
+    """HBM bytes per stage-kernel launch measured by rocprofv3 (FETCH_SIZE and
+    WRITE_SIZE, each scaled by the same counter of a plain copy of known
+    size), read from the committed profile summary of this workload."""
+    path = os.path.join(ROOT, "profiles", f"stage_kernel_{workload_name}.md")
+    try:
+        with open(path) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return int(line.split(":")[1])
+    except OSError:
+        pass
+    return None
+
+
+def aggregate_roofline(cell_steps_by_variant, seconds):
+    """Algorithmic HBM bytes of all stage launches over the timed region."""
+    total = sum(n * bytes_per_cell_step(nv, order) for (nv, order), n in cell_steps_by_variant.items())
+    achieved = total / seconds / 1e9
+    return {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None, "kernel": "stage_kernel (all variants)",
+            "algorithmic_bytes": int(total)}
+
+
+def kapila_bubble(n, mach, R, cx, cy):
+    """Square n x n Kapila shock-bubble on [0,1]^2 (air-helium, reflective
+    top and bottom), shock at x = 0.05 moving +x."""
+    h = 1.0 / n
+    p = H.make_problem(H.KAPILA, n, n, order=2, prim=True, domain=(0.0, 1.0, 0.0, 1.0), cfl=0.5,
+                       bc=(H.BC_TRANSMISSIVE, H.BC_TRANSMISSIVE, H.BC_REFLECTIVE, H.BC_REFLECTIVE),
+                       gamma1=1.4, gamma2=1.648)
+    U0 = H.initial_state(p, H.IC_SHOCK_BUBBLE, [mach, 0.05, cx, cy, R, 1e-6, 2.0 * h, 1.4, 0.25463, 1.0e5])
+    return p, U0
+
+
+def progress(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def run_batched(args):
+    """64 Kapila instances of uneven size advanced together (hype_batch), against
+    the same instances solved one after another (hype_ctx)."""
+    rng = np.random.default_rng(args.seed)
+    count = args.instances
+    steps = args.steps if args.steps is not None else 400
+    sides = np.exp(rng.uniform(np.log(128), np.log(1024), count)).astype(int)
+    insts = [kapila_bubble(int(n), rng.uniform(1.1, 2.0), rng.uniform(0.1, 0.3),
+                           0.5 + rng.uniform(-0.1, 0.1), 0.5 + rng.uniform(-0.1, 0.1)) for n in sides]
+    cells = sum(p.nx * p.ny for p, _ in insts)
+    progress(f"batched: {count} instances, {cells} cells, {steps} steps")
+    with H.Batch([p for p, _ in insts]) as b:
+        for i, (_, U0) in enumerate(insts):
+            b.set_state(i, U0)
+        b.begin(max_steps=min(args.warmup, steps) or 1)
+        b.advance(max(args.warmup, 1))
+        b.sync()
+        for i, (_, U0) in enumerate(insts):
+            b.set_state(i, U0)
+        b.begin(max_steps=steps)
+        b.sync()
+        ms = b.advance_timed(steps)
+        st = b.sync()
+        rows, ntiles, groups = b.tiling()
+        check = [b.get_state(i) for i in range(min(count, args.check))]
+    if any(s.steps != steps for s in st):
+        raise SystemExit("batched: not every instance ran all steps")
+    progress(f"batched: {ms:.1f} ms; now one at a time")
+    # one at a time: each instance on its own context, device-resident, graphs
+    solo_ms = 0.0
+    identical = True
+    for i, (p, U0) in enumerate(insts):
+        with H.Context(p) as ctx:
+            ctx.set_state(U0)
+            ctx.begin(0.0, float("inf"), steps, H.TS_CLIP | H.TS_GRAPH)
+            ctx.sync()
+            solo_ms += ctx.advance_timed(steps)
+            ctx.sync()
+            if i < len(check):
+                identical &= bool(np.array_equal(ctx.get_state(), check[i]))
+    sec = ms / 1e3
+    value = cells * steps / sec / 1e6
+    out = {
+        "metric": METRIC, "value": round(value, 3), "unit": "Mcell-updates/s", "n_gpus": 1, "steps": steps,
+        "warmup": args.warmup, "ms_per_step": round(ms / steps, 5), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"batched_kapila_{count}x_loguniform_128_1024", "instances": count,
+                   "cells": cells, "order": 2, "seed": args.seed, "tile_rows": rows, "tiles": ntiles,
+                   "variants": groups},
+        "roofline": aggregate_roofline({(6, 2): cells * steps}, sec),
+        "one_at_a_time": {"value": round(cells * steps / (solo_ms / 1e3) / 1e6, 3), "unit": "Mcell-updates/s",
+                          "ms": round(solo_ms, 3)},
+        "speedup_vs_one_at_a_time": round(solo_ms / ms, 3),
+        "identical_to_solo": identical, "checked_instances": len(check),
+    }
+    print(json.dumps(out))
+
+
+def served_trace(rng, count, rate):
+    """Poisson arrivals; mix 50% Euler / 35% Kapila / 15% Baer-Nunziato; side
+    log-uniform on [64, 2048]; 100-2000 steps turned into an end time."""
+    t = np.cumsum(rng.exponential(1.0 / rate, count))
+    reqs = []
+    for k in range(count):
+        n = int(np.exp(rng.uniform(np.log(64), np.log(2048))))
+        steps = int(rng.integers(100, 2001))
+        u = rng.uniform()
+        h = 1.0 / n
+        if u < 0.5:
+            p = H.make_problem(H.EULER, n, n, order=2, prim=True, domain=(0.0, 1.0, 0.0, 1.0), cfl=0.5,
+                               bc=(H.BC_TRANSMISSIVE,) * 4, gamma1=1.4)
+            U0 = H.initial_state(p, H.IC_QUADRANTS, [0.5, 0.5, 1.5, 0.0, 0.0, 1.5, 0.5323, 1.206, 0.0, 0.3,
+                                                     0.138, 1.206, 1.206, 0.029, 0.5323, 0.0, 1.206, 0.3])
+        elif u < 0.85:
+            p, U0 = kapila_bubble(n, 1.22, 0.25, 0.5, 0.5)
+        else:
+            p = H.make_problem(H.BAER_NUNZIATO, n, n, order=2, prim=False, domain=(0.0, 1.0, 0.0, 1.0),
+                               cfl=0.5, bc=(H.BC_TRANSMISSIVE,) * 4, gamma1=4.4, gamma2=1.4, pi1=6.0e8)
+            U0 = H.initial_state(p, H.IC_WATER_AIR, [0.5, 1e-6, 2.0 * h, 0.5, 0.3, 0.1, 1.0e9, 1.0e5, 1000.0,
+                                                     1.0, 8, 0.005, 1.0], seed=int(rng.integers(1 << 62)))
+        t_end = steps * dt_estimate(p, U0)
+        reqs.append((float(t[k]), p, U0, t_end, 2 * steps))
+    return reqs
+
+
+def run_served(args):
+    """Requests arrive as a Poisson stream and are served with continuous
+    batching (hype_server); time-to-solution p50/p99 and aggregate throughput,
+    against the same requests served one at a time in arrival order."""
+    rng = np.random.default_rng(args.seed)
+    reqs = served_trace(rng, args.requests, args.rate)
+    progress(f"served: trace of {len(reqs)} requests built")
+    with H.Server() as srv:
+        # warm-up: one small request of each system (kernel loading, graph capture)
+        for _, p, U0, _, _ in reqs[:3]:
+            srv.submit(p, U0, max_steps=2)
+        srv.drain()
+        base = srv.now()
+        ids = []
+        k = 0
+        last = 0.0
+        while k < len(reqs) or srv.poll() != (0, 0):
+            now = srv.now() - base
+            if now - last > 10.0:
+                last = now
+                progress(f"served: t={now:.1f}s submitted={k}")
+            while k < len(reqs) and reqs[k][0] <= now:
+                _, p, U0, t_end, cap = reqs[k]
+                ids.append(srv.submit(p, U0, t_end=t_end, max_steps=cap))
+                k += 1
+            if k < len(reqs) and srv.poll() == (0, 0):
+                time.sleep(min(max(reqs[k][0] - (srv.now() - base), 0.0), 0.01))
+        makespan = srv.now() - base
+        res = [srv.result(i) for i in ids]
+    tts = np.array([inf.t_done - base - r[0] for (_, inf), r in zip(res, reqs)])
+    cell_steps = sum(inf.status.steps * r[1].nx * r[1].ny for (_, inf), r in zip(res, reqs))
+    variants = {}
+    for (_, inf), r in zip(res, reqs):
+        key = (H.NVAR[r[1].system], 2)
+        variants[key] = variants.get(key, 0) + inf.status.steps * r[1].nx * r[1].ny
+    # one at a time, FIFO in arrival order: each request alone on its own
+    # context (upload, solve, download), queueing simulated from its duration
+    progress(f"served: makespan {makespan:.2f}s; now one at a time")
+    identical = True
+    finish, solo_tts, busy = 0.0, [], 0.0
+    for j, ((U, inf), (arr, p, U0, t_end, cap)) in enumerate(zip(res, reqs)):
+        if j % 8 == 0:
+            progress(f"served: one at a time {j}/{len(reqs)}")
+        t0 = time.perf_counter()
+        with H.Context(p) as ctx:
+            ctx.set_state(U0)
+            ctx.begin(0.0, t_end, cap, H.TS_CLIP | H.TS_GRAPH)
+            ctx.advance(cap // 2)  # the requested step count; more if t_end is not reached
+            st = ctx.sync()
+            while not st.done:
+                ctx.advance(32)
+                st = ctx.sync()
+            Us = ctx.get_state()
+        dur = time.perf_counter() - t0
+        busy += dur
+        identical &= bool(np.array_equal(Us, U)) and st.steps == inf.status.steps
+        finish = max(finish, arr) + dur
+        solo_tts.append(finish - arr)
+    solo_tts = np.array(solo_tts)
+    value = cell_steps / makespan / 1e6
+    out = {
+        "metric": METRIC, "value": round(value, 3), "unit": "Mcell-updates/s", "n_gpus": 1,
+        "steps": int(sum(inf.status.steps for _, inf in res)), "warmup": 3, "ms_per_step": None,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"served_poisson_{args.rate:g}rps_{args.requests}req_mix_E50_K35_B15_side64_2048",
+                   "requests": args.requests, "rate_per_s": args.rate, "seed": args.seed,
+                   "cell_steps": int(cell_steps)},
+        "roofline": aggregate_roofline(variants, makespan),
+        "latency_s": {"p50": round(float(np.percentile(tts, 50)), 4), "p99": round(float(np.percentile(tts, 99)), 4),
+                      "makespan": round(makespan, 3)},
+        "one_at_a_time": {"value": round(cell_steps / busy / 1e6, 3), "unit": "Mcell-updates/s",
+                          "p50": round(float(np.percentile(solo_tts, 50)), 4),
+                          "p99": round(float(np.percentile(solo_tts, 99)), 4),
+                          "makespan": round(float(finish), 3)},
+        "identical_to_solo": identical,
+    }
+    print(json.dumps(out))
 
 
 def calibrate_copy(mib):
@@ -120,6 +357,11 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--graph", type=int, default=1)
     ap.add_argument("--hrows", type=int, default=0)
+    ap.add_argument("--instances", type=int, default=64, help="batched: number of instances")
+    ap.add_argument("--check", type=int, default=64, help="batched: instances compared with solo runs")
+    ap.add_argument("--requests", type=int, default=64, help="served: number of requests of the trace")
+    ap.add_argument("--rate", type=float, default=20.0, help="served: Poisson arrival rate per second")
+    ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="run only a plain device copy of MIB MiB (counter calibration)")
     args = ap.parse_args()
@@ -130,6 +372,12 @@ def main():
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is a design only "
                                    "(see DESIGN.md), so --gpus must be 1"}))
         sys.exit(2)
+    if args.config == "batched":
+        run_batched(args)
+        return
+    if args.config == "served":
+        run_served(args)
+        return
 
     p, U0, steps_default, wname = workload(args.config)
     steps = args.steps if args.steps is not None else steps_default
@@ -150,6 +398,14 @@ def main():
         ms = ctx.advance_timed(steps)
         st = ctx.sync()
         hrows, ntiles, grid = ctx.tiling()
+        # the drop-in callback with host buffers (TSRHSFunction on host Vecs):
+        # every call uploads U and downloads the RHS over PCIe
+        ctx.rhs(U0)
+        nrhs = 10
+        t0 = time.perf_counter()
+        for _ in range(nrhs):
+            ctx.rhs(U0)
+        rhs_s = (time.perf_counter() - t0) / nrhs
     if st.steps != steps:
         raise SystemExit(f"only {st.steps} of {steps} steps ran")
     sec = ms / 1e3
@@ -177,10 +433,13 @@ def main():
                    "scheme": "WENO3+Rusanov+SSPRK(4,3)" if p.order == 2 else f"order{p.order}",
                    "cfl": p.cfl, "tile_rows": hrows, "tiles": ntiles, "graph": bool(args.graph)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": load_traffic(wname),
                      "kernel": "stage_kernel", "avg_launch_us": round(avg_launch_s * 1e6, 3),
                      "algorithmic_bytes_per_launch": int(bytes_launch)},
         "final_time": st.t,
+        "pcie_inclusive": {"rhs_call_ms": round(rhs_s * 1e3, 4), "nstage": nstage,
+                           "equivalent_value": round(cells / (rhs_s * nstage) / 1e6, 3),
+                           "note": "hype_rhs on host buffers (upload U, download RHS), per RK stage"},
     }
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(p, U0, nv)
