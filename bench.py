@@ -246,15 +246,12 @@ def served_trace(rng, count, rate):
     return reqs
 
 
-def run_served(args):
-    """Requests arrive as a Poisson stream and are served with continuous
-    batching (hype_server); time-to-solution p50/p99 and aggregate throughput,
-    against the same requests served one at a time in arrival order."""
-    rng = np.random.default_rng(args.seed)
-    reqs = served_trace(rng, args.requests, args.rate)
-    progress(f"served: trace of {len(reqs)} requests built")
-    with H.Server() as srv:
-        # warm-up: one small request of each system (kernel loading, graph capture)
+def serve_trace(reqs, policy=H.SERVE_FIFO, active_cells=0):
+    """Replay a trace through one hype_server in real time; returns the
+    results (U, info) in submission order, the server clock at the first
+    arrival and the makespan."""
+    with H.Server(policy=policy, max_active_cells=active_cells) as srv:
+        # warm-up: a few tiny requests (kernel loading, graph capture)
         for _, p, U0, _, _ in reqs[:3]:
             srv.submit(p, U0, max_steps=2)
         srv.drain()
@@ -275,6 +272,20 @@ def run_served(args):
                 time.sleep(min(max(reqs[k][0] - (srv.now() - base), 0.0), 0.01))
         makespan = srv.now() - base
         res = [srv.result(i) for i in ids]
+    return res, base, makespan
+
+
+POLICIES = {"fifo": H.SERVE_FIFO, "shortest": H.SERVE_SHORTEST}
+
+
+def run_served(args):
+    """Requests arrive as a Poisson stream and are served with continuous
+    batching (hype_server); time-to-solution p50/p99 and aggregate throughput,
+    against the same requests served one at a time in arrival order."""
+    rng = np.random.default_rng(args.seed)
+    reqs = served_trace(rng, args.requests, args.rate)
+    progress(f"served: trace of {len(reqs)} requests built")
+    res, base, makespan = serve_trace(reqs, POLICIES[args.policy], args.active_cells)
     tts = np.array([inf.t_done - base - r[0] for (_, inf), r in zip(res, reqs)])
     cell_steps = sum(inf.status.steps * r[1].nx * r[1].ny for (_, inf), r in zip(res, reqs))
     variants = {}
@@ -312,6 +323,7 @@ def run_served(args):
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": f"served_poisson_{args.rate:g}rps_{args.requests}req_mix_E50_K35_B15_side64_2048",
                    "requests": args.requests, "rate_per_s": args.rate, "seed": args.seed,
+                   "policy": args.policy, "max_active_cells": args.active_cells,
                    "cell_steps": int(cell_steps)},
         "roofline": aggregate_roofline(variants, makespan),
         "latency_s": {"p50": round(float(np.percentile(tts, 50)), 4), "p99": round(float(np.percentile(tts, 99)), 4),
@@ -361,6 +373,9 @@ def main():
     ap.add_argument("--check", type=int, default=64, help="batched: instances compared with solo runs")
     ap.add_argument("--requests", type=int, default=64, help="served: number of requests of the trace")
     ap.add_argument("--rate", type=float, default=20.0, help="served: Poisson arrival rate per second")
+    ap.add_argument("--policy", choices=sorted(POLICIES), default="fifo", help="served: admission order")
+    ap.add_argument("--active-cells", type=int, default=0,
+                    help="served: cap on cells of running instances (0 = fill the device, -1 = none)")
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="run only a plain device copy of MIB MiB (counter calibration)")

@@ -46,6 +46,7 @@ struct Nrm {
 // ------------------------------------------------------------------ Euler
 struct Euler {
   static constexpr int NV = 4;
+  static constexpr int WAVES_PER_EU = 2;  // register budget of the stage kernel: 256 per lane
   static constexpr int REFX[2] = {1, -1};  // reflective x-wall negates comp 1
   static constexpr int REFY[2] = {2, -1};
   static constexpr bool HAS_NCP = false;
@@ -142,6 +143,7 @@ struct Euler {
 // ----------------------------------------------------------------- Kapila
 struct Kapila {
   static constexpr int NV = 6;
+  static constexpr int WAVES_PER_EU = 1;
   static constexpr int REFX[2] = {2, -1};
   static constexpr int REFY[2] = {3, -1};
   static constexpr bool HAS_NCP = true;
@@ -361,6 +363,7 @@ struct Kapila {
 // ---------------------------------------------------------- Baer-Nunziato
 struct BN {
   static constexpr int NV = 9;
+  static constexpr int WAVES_PER_EU = 1;
   static constexpr int REFX[2] = {1, 5};
   static constexpr int REFY[2] = {2, 6};
   static constexpr bool HAS_NCP = true;

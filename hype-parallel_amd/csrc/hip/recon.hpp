@@ -11,6 +11,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include "fdiv.hpp"
+
 namespace hype {
 
 constexpr double GPT = 0.28867513459481287;  // hype.c:214-230
@@ -102,6 +104,94 @@ __device__ __forceinline__ double pow4(double a) {  // reconstruction.c:123-127
   return a2 * a2;
 }
 
+// weno3 with every division done by fdiv.hpp's shared-reciprocal sequence.
+// Same expressions and operation order as the reference weno3
+// (reconstruction.c:144-221), so the coefficients are bitwise those of the
+// IEEE-division version whenever all divisors are in range; returns false
+// otherwise (the caller then recomputes with plain division).  The divisors
+// are pow4(IS + 1e-12) >= 1e-48 and the weight sums; a group is in range when
+// the sum of its pow4 terms is at most 2^380 (then every weight and sum lies
+// in [2^-380, 2^380]).
+__device__ __forceinline__ bool weno3_fast(const double* U_x, const double* U_y, const double* U_xy,
+                                           double* coeffs) {
+  const double central_cell_wt = 100.0;
+  const double r13_3 = 13. / 3.;
+  double u_0 = U_xy[0];
+  double u_ip1 = U_x[3], u_jp1 = U_y[3], u_ip1jp1 = U_xy[1];
+  double u_im1 = U_x[1], u_jm1 = U_y[1], u_ip1jm1 = U_xy[2];
+  double u_ip2 = U_x[4], u_jp2 = U_y[4], u_im1jp1 = U_xy[3];
+  double u_im2 = U_x[0], u_jm2 = U_y[0], u_im1jm1 = U_xy[4];
+  double ux0, ux1, ux2, uxx0, uxx1, uxx2, d0, d1, d2, d3, w0, w1, w2, w3, total, yt, temp;
+  bool ok;
+
+  ux0 = -2.0 * u_im1 + 0.5 * u_im2 + 1.5 * u_0;
+  uxx0 = 0.5 * (u_im2 - 2.0 * u_im1 + u_0);
+  ux1 = 0.5 * (u_ip1 - u_im1);
+  uxx1 = 0.5 * (u_im1 - 2.0 * u_0 + u_ip1);
+  ux2 = -1.5 * u_0 + 2.0 * u_ip1 - 0.5 * u_ip2;
+  uxx2 = 0.5 * (u_0 - 2.0 * u_ip1 + u_ip2);
+  d0 = pow4(ux0 * ux0 + r13_3 * uxx0 * uxx0 + SMALL_NUM);
+  d1 = pow4(ux1 * ux1 + r13_3 * uxx1 * uxx1 + SMALL_NUM);
+  d2 = pow4(ux2 * ux2 + r13_3 * uxx2 * uxx2 + SMALL_NUM);
+  ok = (d0 + d1 + d2) <= FDIV_HI;
+  w0 = inv_y(d0, recip_nr(d0));
+  w1 = div_y(central_cell_wt, d1, recip_nr(d1));
+  w2 = inv_y(d2, recip_nr(d2));
+  total = w0 + w1 + w2;
+  yt = recip_nr(total);
+  w0 = div_y(w0, total, yt);
+  w1 = div_y(w1, total, yt);
+  w2 = div_y(w2, total, yt);
+  coeffs[0] = u_0;
+  coeffs[1] = w0 * ux0 + w1 * ux1 + w2 * ux2;
+  coeffs[3] = w0 * uxx0 + w1 * uxx1 + w2 * uxx2;
+
+  ux0 = -2.0 * u_jm1 + 0.5 * u_jm2 + 1.5 * u_0;
+  uxx0 = 0.5 * (u_jm2 - 2.0 * u_jm1 + u_0);
+  ux1 = 0.5 * (u_jp1 - u_jm1);
+  uxx1 = 0.5 * (u_jm1 - 2.0 * u_0 + u_jp1);
+  ux2 = -1.5 * u_0 + 2.0 * u_jp1 - 0.5 * u_jp2;
+  uxx2 = 0.5 * (u_0 - 2.0 * u_jp1 + u_jp2);
+  d0 = pow4(ux0 * ux0 + r13_3 * uxx0 * uxx0 + SMALL_NUM);
+  d1 = pow4(ux1 * ux1 + r13_3 * uxx1 * uxx1 + SMALL_NUM);
+  d2 = pow4(ux2 * ux2 + r13_3 * uxx2 * uxx2 + SMALL_NUM);
+  ok = ok && (d0 + d1 + d2) <= FDIV_HI;
+  w0 = inv_y(d0, recip_nr(d0));
+  w1 = div_y(central_cell_wt, d1, recip_nr(d1));
+  w2 = inv_y(d2, recip_nr(d2));
+  total = w0 + w1 + w2;
+  yt = recip_nr(total);
+  w0 = div_y(w0, total, yt);
+  w1 = div_y(w1, total, yt);
+  w2 = div_y(w2, total, yt);
+  coeffs[2] = w0 * ux0 + w1 * ux1 + w2 * ux2;
+  coeffs[4] = w0 * uxx0 + w1 * uxx1 + w2 * uxx2;
+
+  double c1 = coeffs[1], c2 = coeffs[2], c3 = coeffs[3], c4 = coeffs[4];
+  double uxy0 = u_ip1jp1 - u_0 - c1 - c2 - c3 - c4;
+  double uxy1 = -u_ip1jm1 + u_0 + c1 - c2 + c3 + c4;
+  double uxy2 = -u_im1jp1 + u_0 - c1 + c2 + c3 + c4;
+  double uxy3 = u_im1jm1 - u_0 + c1 + c2 - c3 - c4;
+  temp = 4.0 * (c3 * c3 + c4 * c4);
+  d0 = pow4(temp + uxy0 * uxy0 + SMALL_NUM);
+  d1 = pow4(temp + uxy1 * uxy1 + SMALL_NUM);
+  d2 = pow4(temp + uxy2 * uxy2 + SMALL_NUM);
+  d3 = pow4(temp + uxy3 * uxy3 + SMALL_NUM);
+  ok = ok && (d0 + d1 + d2 + d3) <= FDIV_HI;
+  w0 = inv_y(d0, recip_nr(d0));
+  w1 = inv_y(d1, recip_nr(d1));
+  w2 = inv_y(d2, recip_nr(d2));
+  w3 = inv_y(d3, recip_nr(d3));
+  total = w0 + w1 + w2 + w3;
+  yt = recip_nr(total);
+  w0 = div_y(w0, total, yt);
+  w1 = div_y(w1, total, yt);
+  w2 = div_y(w2, total, yt);
+  w3 = div_y(w3, total, yt);
+  coeffs[5] = w0 * uxy0 + w1 * uxy1 + w2 * uxy2 + w3 * uxy3;
+  return ok;
+}
+
 // Stencil of one component: x[0..4] = u(i-2..i+2, j), y[0..4] = u(i, j-2..j+2),
 // d[0..4] = u(i,j), u(i+1,j+1), u(i+1,j-1), u(i-1,j+1), u(i-1,j-1)
 // (rhs_function_prim.c:332-348).
@@ -117,6 +207,7 @@ __device__ __forceinline__ void reconstruct(const double* U_x, const double* U_y
     coeffs[1] = minmod(U_x[3] - u_0, u_0 - U_x[1]);
     coeffs[2] = minmod(U_y[3] - u_0, u_0 - U_y[1]);
   } else {
+    if (__builtin_expect(weno3_fast(U_x, U_y, U_xy, coeffs), 1)) return;
     // reconstruction.c:144-221
     const double central_cell_wt = 100.0;
     const double r13_3 = 13. / 3.;

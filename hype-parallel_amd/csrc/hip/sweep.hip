@@ -300,15 +300,17 @@ struct Tile {
         ud[4] = W_(sm1, c, cw - 1);
         reconstruct<N>(ux, uy, ud, co[c]);
       }
+      // values at the nodes, which are also the face quadrature points
+      // (rhs_function_prim.c:382-388, 425-440; the same points, hype.c:207-229)
+      double fv[NNODE][NV];
+#pragma unroll
+      for (int p = 0; p < NNODE; ++p)
+#pragma unroll
+        for (int c = 0; c < NV; ++c) fv[p][c] = at_node(co, c, p);
       // admissibility at the nodes (rhs_function_prim.c:393-404)
       bool pad = true;
 #pragma unroll
-      for (int p = 0; p < NNODE; ++p) {
-        double V[NV];
-#pragma unroll
-        for (int c = 0; c < NV; ++c) V[c] = at_node(co, c, p);
-        pad = pad && (PRIM ? S::pad_prim(eos, V) : S::pad(eos, V));
-      }
+      for (int p = 0; p < NNODE; ++p) pad = pad && (PRIM ? S::pad_prim(eos, fv[p]) : S::pad(eos, fv[p]));
       if (!pad) {  // rhs_function_prim.c:407-421 / rhs_function.c:409-421
 #pragma unroll
         for (int c = 0; c < NV; ++c) {
@@ -321,6 +323,10 @@ struct Tile {
 #pragma unroll
           for (int k = 3; k < NDOF; ++k) co[c][k] = 0.0;
         }
+#pragma unroll
+        for (int p = 0; p < NNODE; ++p)
+#pragma unroll
+          for (int c = 0; c < NV; ++c) fv[p][c] = at_node(co, c, p);
       }
       if (L.fallback && ownedRow && owned) L.fallback[(size_t)r * nx + col] = (unsigned char)(!pad);
 
@@ -350,12 +356,6 @@ struct Tile {
         }
       }
 
-      // ---- values at the face quadrature points (u_bnd, :425-440) -------
-      double fv[NNODE][NV];
-#pragma unroll
-      for (int p = 0; p < NNODE; ++p)
-#pragma unroll
-        for (int c = 0; c < NV; ++c) fv[p][c] = at_node(co, c, p);
 
       // ---- x-faces i-1/2 (rhs_function_prim.c:485-533) -------------------
       double xpart[NV];
@@ -491,7 +491,8 @@ struct Tile {
 };
 
 template <class S, bool PRIM, int N>
-__global__ __launch_bounds__(HYPE_WAVE) void stage_kernel(hype_launch L) {
+__global__ __launch_bounds__(HYPE_WAVE) __attribute__((amdgpu_waves_per_eu(S::WAVES_PER_EU))) void stage_kernel(
+    hype_launch L) {
   __shared__ double win[NSLOT * S::NV * COLS];
   const int lane = threadIdx.x;
   const int ntiles = L.ntiles_dev ? *L.ntiles_dev : L.ntiles;

@@ -22,7 +22,8 @@ int eng_init(hype_engine *E, int cap, int target_rows, unsigned flags)
     E->group_of = (int *)calloc((size_t)cap, sizeof(int));
     E->nsy = (int *)calloc((size_t)cap, sizeof(int));
     E->fresh = (int *)calloc((size_t)cap, sizeof(int));
-    if (!E->inst_h || !E->P || !E->used || !E->group_of || !E->nsy || !E->fresh) return HYPE_E_NOMEM;
+    E->run = (int *)calloc((size_t)cap, sizeof(int));
+    if (!E->inst_h || !E->P || !E->used || !E->group_of || !E->nsy || !E->fresh || !E->run) return HYPE_E_NOMEM;
     HYPE_TRY(hyped_malloc((void **)&E->inst_d, (size_t)cap * sizeof(hype_inst)));
     HYPE_TRY(hyped_malloc((void **)&E->st_d, (size_t)cap * sizeof(hype_dstate)));
     HYPE_TRY(hyped_stream_create(&E->stream));
@@ -55,6 +56,7 @@ void eng_free(hype_engine *E)
     free(E->group_of);
     free(E->nsy);
     free(E->fresh);
+    free(E->run);
     memset(E, 0, sizeof(*E));
 }
 
@@ -111,6 +113,7 @@ int eng_attach(hype_engine *E, int slot, const hype_problem *p, double *U, doubl
     E->P[slot] = *p;
     E->group_of[slot] = g;
     E->used[slot] = 1;
+    E->run[slot] = 1;
     E->fresh[slot] = 1;
     hype_fill_inst(&E->inst_h[slot], p, 1);
     E->inst_h[slot].buf[0] = U;
@@ -125,7 +128,15 @@ int eng_detach(hype_engine *E, int slot)
 {
     if (slot < 0 || slot >= E->cap || !E->used[slot]) return HYPE_E_INVALID;
     E->used[slot] = 0;
+    E->run[slot] = 0;
     E->fresh[slot] = 0;
+    return HYPE_OK;
+}
+
+int eng_set_running(hype_engine *E, int slot, int on)
+{
+    if (slot < 0 || slot >= E->cap || !E->used[slot]) return HYPE_E_INVALID;
+    E->run[slot] = on ? 1 : 0;
     return HYPE_OK;
 }
 
@@ -143,7 +154,7 @@ static int count_tiles(const hype_engine *E, int g, int only_fresh)
 {
     int s, t = 0;
     for (s = 0; s < E->cap; ++s)
-        if (E->used[s] && (g < 0 || E->group_of[s] == g) && (!only_fresh || E->fresh[s]))
+        if (E->used[s] && (g < 0 || E->group_of[s] == g) && (only_fresh ? E->fresh[s] : E->run[s]))
             t += hype_strips(E->P[s].nx) * rows_for(E, s);
     return t;
 }
@@ -184,7 +195,7 @@ int eng_pick_rows(hype_engine *E)
         long tiles = 0;
         int s;
         for (s = 0; s < E->cap; ++s)
-            if (E->used[s]) {
+            if (E->used[s] && E->run[s]) {
                 int ny = E->P[s].ny, nsy = (ny + h - 1) / h;
                 tiles += (long)hype_strips(E->P[s].nx) * nsy;
             }
@@ -223,7 +234,7 @@ static int build_tiles(hype_engine *E, int g, int only_fresh, hype_tile *buf, in
     for (s = 0; s < E->cap; ++s) {
         int k, m;
         if (!E->used[s] || E->group_of[s] != g) continue;
-        if (only_fresh && !E->fresh[s]) continue;
+        if (only_fresh ? !E->fresh[s] : !E->run[s]) continue;
         E->nsy[s] = rows_for(E, s);
         E->inst_h[s].nsy = E->nsy[s];
         m = hype_make_tiles(s, E->P[s].nx, E->P[s].ny, hype_strips(E->P[s].nx), E->nsy[s], buf + n, cap - n);

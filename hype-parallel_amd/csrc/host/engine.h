@@ -39,6 +39,7 @@ typedef struct {
     hype_dstate *st_d;      /* one per slot */
     hype_problem *P;
     int *used, *group_of, *nsy, *fresh;
+    int *run;               /* used slots advanced by eng_step (default all) */
     int target_rows;        /* common tile height of all instances */
     eng_group g[HYPE_MAX_GROUPS];
     int ng;
@@ -54,7 +55,10 @@ void eng_free(hype_engine *E);
 /* place an instance in a slot (its U, A, B device buffers) */
 int eng_attach(hype_engine *E, int slot, const hype_problem *p, double *U, double *A, double *B);
 int eng_detach(hype_engine *E, int slot);
-/* recompute the tile tables of every group from the used slots */
+/* include (on = 1) or leave out a used slot from the steps; takes effect at
+ * the next eng_commit.  A left-out instance keeps its state and time. */
+int eng_set_running(hype_engine *E, int slot, int on);
+/* recompute the tile tables of every group from the running slots */
 int eng_commit(hype_engine *E);
 /* initialise the time state of the fresh slots and run their priming call */
 int eng_start(hype_engine *E, const int *slots, int n, const double *t0, const double *t_end,

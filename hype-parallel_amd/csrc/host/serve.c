@@ -9,9 +9,13 @@
  *   1. reads the time state of the running instances (one small copy),
  *   2. retires the finished ones: result to the caller's buffer, slot and
  *      HBM block freed,
- *   3. admits queued requests in arrival order while slots and budget allow,
- *   4. rebuilds the tile tables and primes the new instances, and
- *   5. enqueues the next steps of everything running: one persistent launch
+ *   3. admits queued requests while slots and the HBM budget allow, in
+ *      arrival order or least work (cells x nvar x step budget) first,
+ *   4. picks the instances advanced this round: resident ones in arrival
+ *      order, or least remaining work first, up to a cap on their cells
+ *      (the others keep their state in HBM and wait),
+ *   5. rebuilds the tile tables and primes the new instances, and
+ *   6. enqueues the next steps of the running set: one persistent launch
  *      per RK stage per kernel variant, replayed from a captured graph.
  * An instance's arithmetic does not depend on what else runs: results are
  * bitwise those of a solo hype_ctx solve.
@@ -55,8 +59,13 @@ struct hype_server {
     arena A;
     srv_req *req;
     int nreq, capreq;
-    int next;               /* first request not yet admitted (FIFO) */
-    int *slot_req;          /* request running in each slot, or -1 */
+    int *queue;             /* ids of queued requests, in arrival order */
+    int nqueue;
+    long long active_cells;
+    int *slot_req;          /* request resident in each slot, or -1 */
+    int *is_new;            /* admitted this round (time state not read yet) */
+    int *order;             /* scratch for the running-set selection */
+    double *key;
     int active;
     int steps;              /* steps enqueued by the last round */
     double ms_per_step;     /* measured GPU time of one step of the mix */
@@ -161,7 +170,11 @@ void hype_server_destroy(hype_server *s)
     if (s->ev0) hyped_event_destroy(s->ev0);
     if (s->ev1) hyped_event_destroy(s->ev1);
     free(s->req);
+    free(s->queue);
     free(s->slot_req);
+    free(s->is_new);
+    free(s->order);
+    free(s->key);
     free(s->d);
     free(s);
 }
@@ -178,6 +191,10 @@ int hype_server_create(const hype_server_opts *opts, hype_server **out)
     if (opts) s->o = *opts;
     if (s->o.max_active < 1) s->o.max_active = 512;
     if (s->o.round_ms <= 0.0) s->o.round_ms = 20.0;
+    if (s->o.policy != HYPE_SERVE_FIFO && s->o.policy != HYPE_SERVE_SHORTEST) {
+        free(s);
+        return HYPE_E_INVALID;
+    }
     clock_gettime(CLOCK_MONOTONIC, &s->t0);
     budget = s->o.hbm_budget_bytes;
     if ((rc = hyped_init()) != HYPE_OK) {
@@ -193,9 +210,18 @@ int hype_server_create(const hype_server_opts *opts, hype_server **out)
         budget = fr / 4 * 3;
     }
     s->o.hbm_budget_bytes = budget;
+    if (s->o.max_active_cells == 0) {
+        /* tiles of a few hundred rows for every resident wave of the device:
+         * enough for short halo overhead, more only delays everyone */
+        int ncu = hyped_num_cus();
+        s->o.max_active_cells = (long long)(ncu > 0 ? ncu : 256) * 4 * HYPE_TILE_W * 256;
+    }
     s->slot_req = (int *)malloc((size_t)s->o.max_active * sizeof(int));
-    s->d = (hype_dstate *)malloc((size_t)s->o.max_active * sizeof(hype_dstate));
-    if (!s->slot_req || !s->d) {
+    s->is_new = (int *)calloc((size_t)s->o.max_active, sizeof(int));
+    s->order = (int *)malloc((size_t)s->o.max_active * sizeof(int));
+    s->key = (double *)malloc((size_t)s->o.max_active * sizeof(double));
+    s->d = (hype_dstate *)calloc((size_t)s->o.max_active, sizeof(hype_dstate));
+    if (!s->slot_req || !s->is_new || !s->order || !s->key || !s->d) {
         hype_server_destroy(s);
         return HYPE_E_NOMEM;
     }
@@ -225,8 +251,12 @@ int hype_server_submit(hype_server *s, const hype_request *r, int *id)
     if (s->nreq == s->capreq) {
         int cap = s->capreq ? 2 * s->capreq : 256;
         srv_req *nr = (srv_req *)realloc(s->req, (size_t)cap * sizeof(srv_req));
+        int *nq;
         if (!nr) return HYPE_E_NOMEM;
         s->req = nr;
+        nq = (int *)realloc(s->queue, (size_t)cap * sizeof(int));
+        if (!nq) return HYPE_E_NOMEM;
+        s->queue = nq;
         s->capreq = cap;
     }
     q = &s->req[s->nreq];
@@ -237,8 +267,75 @@ int hype_server_submit(hype_server *s, const hype_request *r, int *id)
     q->bytes = 3 * state_bytes(&r->problem);
     q->t_submit = now_s(s);
     if (id) *id = s->nreq;
+    s->queue[s->nqueue++] = s->nreq;
     ++s->nreq;
     return HYPE_OK;
+}
+
+/* queue position of the next request to admit */
+static int pick(const hype_server *s)
+{
+    int k, best = 0;
+    double bw = 0.0;
+    if (s->o.policy == HYPE_SERVE_FIFO) return 0;
+    for (k = 0; k < s->nqueue; ++k) {
+        const srv_req *q = &s->req[s->queue[k]];
+        double w = (double)q->r.problem.nx * q->r.problem.ny * hype_nvar(q->r.problem.system) * q->r.max_steps;
+        if (k == 0 || w < bw) {
+            bw = w;
+            best = k;
+        }
+    }
+    return best;
+}
+
+static double remaining_work(const hype_server *s, int slot)
+{
+    const srv_req *q = &s->req[s->slot_req[slot]];
+    const hype_dstate *d = &s->d[slot];
+    double steps = q->r.max_steps;
+    if (!s->is_new[slot]) {
+        steps = (double)(d->max_steps - d->step);
+        if (d->dt > 0.0 && d->t_end < 1e300) {
+            const double by_time = (d->t_end - d->t) / d->dt + 1.0;
+            if (by_time < steps) steps = by_time;
+        }
+    }
+    return (double)q->r.problem.nx * q->r.problem.ny * hype_nvar(q->r.problem.system) * steps;
+}
+
+static int cmp_key(const void *a, const void *b, void *ctx)
+{
+    const double *key = (const double *)ctx;
+    const int x = *(const int *)a, y = *(const int *)b;
+    if (key[x] < key[y]) return -1;
+    if (key[x] > key[y]) return 1;
+    return x - y;
+}
+
+/* choose the running subset; returns 1 if it changed */
+static int select_running(hype_server *s)
+{
+    int k, n = 0, changed = 0;
+    long long cells = 0;
+    for (k = 0; k < s->o.max_active; ++k) {
+        if (s->slot_req[k] < 0) continue;
+        s->order[n++] = k;
+        s->key[k] = s->o.policy == HYPE_SERVE_SHORTEST ? remaining_work(s, k) : (double)s->slot_req[k];
+    }
+    qsort_r(s->order, (size_t)n, sizeof(int), cmp_key, s->key);
+    for (k = 0; k < n; ++k) {
+        const int slot = s->order[k];
+        const srv_req *q = &s->req[s->slot_req[slot]];
+        const long long c = (long long)q->r.problem.nx * q->r.problem.ny;
+        const int on = k == 0 || s->o.max_active_cells < 0 || cells + c <= s->o.max_active_cells;
+        if (on) cells += c;
+        if (on != s->E.run[slot]) {
+            eng_set_running(&s->E, slot, on);
+            changed = 1;
+        }
+    }
+    return changed;
 }
 
 static int finished(const hype_dstate *d, unsigned long long *err)
@@ -269,6 +366,7 @@ static int retire(hype_server *s, int slot, double t)
     HYPE_TRY(eng_detach(&s->E, slot));
     HYPE_TRY(arena_release(&s->A, q->off, q->bytes));
     s->slot_req[slot] = -1;
+    s->active_cells -= (long long)q->r.problem.nx * q->r.problem.ny;
     --s->active;
     return HYPE_OK;
 }
@@ -330,10 +428,12 @@ int hype_server_poll(hype_server *s, int *active, int *queued)
             changed = 1;
         }
     }
-    /* 3: admit in arrival order while a slot and HBM are free */
+    /* 3: admit while a slot and HBM are free */
     t = now_s(s);
-    while (s->next < s->nreq) {
-        srv_req *q = &s->req[s->next];
+    while (s->nqueue > 0) {
+        int k = pick(s), id = s->queue[k];
+        srv_req *q = &s->req[id];
+        long long cells = (long long)q->r.problem.nx * q->r.problem.ny;
         double *U;
         size_t m;
         for (slot = 0; slot < s->o.max_active && s->slot_req[slot] >= 0; ++slot) {
@@ -341,7 +441,7 @@ int hype_server_poll(hype_server *s, int *active, int *queued)
         if (slot == s->o.max_active) break;
         if (arena_alloc(&s->A, q->bytes, &q->off) != 0) break;
         U = (double *)(s->A.base + q->off);
-        m = (size_t)q->r.problem.nx * q->r.problem.ny * hype_nvar(q->r.problem.system);
+        m = (size_t)cells * hype_nvar(q->r.problem.system);
         if ((rc = hyped_memcpy_h2d(U, q->r.U0, m * sizeof(double), s->E.stream)) != HYPE_OK ||
             (rc = eng_attach(&s->E, slot, &q->r.problem, U, U + m, U + 2 * m)) != HYPE_OK) {
             arena_release(&s->A, q->off, q->bytes);
@@ -350,20 +450,26 @@ int hype_server_poll(hype_server *s, int *active, int *queued)
         q->slot = slot;
         q->state = HYPE_REQ_RUNNING;
         q->t_start = t;
-        s->slot_req[slot] = s->next;
+        s->slot_req[slot] = id;
+        s->is_new[slot] = 1;
         fresh[nfresh++] = slot;
         ++s->active;
-        ++s->next;
+        s->active_cells += cells;
+        memmove(s->queue + k, s->queue + k + 1, (size_t)(s->nqueue - k - 1) * sizeof(int));
+        --s->nqueue;
         changed = 1;
     }
-    /* 4: new tile tables, priming of the admitted instances */
+    /* 4: the instances advanced this round */
+    if (s->active > 0 && select_running(s)) changed = 1;
+    /* 5: new tile tables, priming of the admitted instances */
     if (changed && s->active > 0) {
         if (s->o.tile_rows < 1) eng_set_rows(&s->E, eng_pick_rows(&s->E));
         if ((rc = eng_commit(&s->E)) != HYPE_OK) goto out;
         if ((rc = start_fresh(s, fresh, nfresh, 1)) != HYPE_OK) goto out;
         if ((rc = start_fresh(s, fresh, nfresh, 0)) != HYPE_OK) goto out;
     }
-    /* 5: the next steps of everything running */
+    for (slot = 0; slot < nfresh; ++slot) s->is_new[fresh[slot]] = 0;
+    /* 6: the next steps of the running set */
     s->steps = 0;
     s->timing = 0;
     if (s->active > 0) {
@@ -382,7 +488,7 @@ int hype_server_poll(hype_server *s, int *active, int *queued)
 out:
     free(fresh);
     if (active) *active = s->active;
-    if (queued) *queued = s->nreq - s->next;
+    if (queued) *queued = s->nqueue;
     return rc;
 }
 

@@ -69,7 +69,12 @@ class ServerOpts(C.Structure):
         ("steps_per_round", C.c_int),
         ("tile_rows", C.c_int),
         ("round_ms", C.c_double),
+        ("max_active_cells", C.c_longlong),
+        ("policy", C.c_int),
     ]
+
+
+SERVE_FIFO, SERVE_SHORTEST = 0, 1
 
 
 class Request(C.Structure):
@@ -447,9 +452,10 @@ class Batch:
 class Server:
     """Request loop with continuous batching (hype_server)."""
 
-    def __init__(self, hbm_budget_bytes=0, max_active=0, steps_per_round=0, tile_rows=0, round_ms=0.0):
+    def __init__(self, hbm_budget_bytes=0, max_active=0, steps_per_round=0, tile_rows=0, round_ms=0.0,
+                 max_active_cells=0, policy=SERVE_FIFO):
         self.L = lib()
-        o = ServerOpts(hbm_budget_bytes, max_active, steps_per_round, tile_rows, round_ms)
+        o = ServerOpts(hbm_budget_bytes, max_active, steps_per_round, tile_rows, round_ms, max_active_cells, policy)
         h = C.c_void_p()
         rc = self.L.hype_server_create(C.byref(o), C.byref(h))
         if rc != OK:

@@ -127,8 +127,9 @@ int hype_get_state(hype_ctx *c, double *U, int mem);
 
 /* Start a solve on the current state: priming RHS call for dt (hype.c:343,
  * 356), time t0, stop at t_end or after max_steps steps.  The scheme is the
- * one the reference selects for order N (hype.c:362-382): N=0 forward Euler,
- * N=1 SSPRK(2,2), N=2 SSPRK(4,3).  dt of step k+1 comes from the last RHS
+ * one the reference selects for order N (hype.c:362-382): N=1 SSPRK(2,2),
+ * N=2 SSPRK(4,3); N=0 (TSEULER) is not stepped on the device and returns
+ * HYPE_E_UNSUPPORTED (hype_rhs supports it).  dt of step k+1 comes from the last RHS
  * evaluation of step k (MonitorFunction, post_process.c:19), computed and
  * consumed on the device. */
 int hype_begin(hype_ctx *c, double t0, double t_end, int max_steps,
@@ -228,9 +229,10 @@ int hype_batch_autotune(hype_batch *b, const int *cand, int ncand, int steps,
 
 /* ======================================================================
  * Served mode: a stream of independent solve requests with continuous
- * batching.  Requests are admitted in arrival order while instance slots
- * and the HBM budget allow; each round retires finished instances, admits
- * queued ones and advances everything running by a few steps.  A request's
+ * batching.  Requests are admitted while instance slots and the HBM budget
+ * allow; each round retires finished instances, admits queued ones and
+ * advances the resident instances that fit an active-cell cap (in arrival
+ * order, or least remaining work first) by a few steps.  A request's
  * result is bitwise the result of the same solve run alone.  The reference
  * has no counterpart (one problem per program run, hype.c:14-456).
  * ====================================================================== */
@@ -244,7 +246,16 @@ typedef struct hype_server_opts {
     int steps_per_round;     /* 0 = as many as fit round_ms */
     int tile_rows;           /* 0 = tile-shape model at every admission */
     double round_ms;         /* target GPU time of a round; 0 = 20 ms */
+    long long max_active_cells; /* cells advanced per round: resident
+                                   instances beyond it wait in HBM (one
+                                   always runs); 0 = enough to fill the
+                                   device, < 0 = no cap */
+    int policy;              /* HYPE_SERVE_FIFO or HYPE_SERVE_SHORTEST */
 } hype_server_opts;
+
+/* Admission and, under the cell cap, the order instances are advanced in */
+#define HYPE_SERVE_FIFO 0     /* arrival order */
+#define HYPE_SERVE_SHORTEST 1 /* least remaining work (cells x nvar x steps) first */
 
 typedef struct hype_request {
     hype_problem problem;    /* order 1 or 2 */

@@ -45,14 +45,19 @@ def state_bytes(p):
     return p.nx * p.ny * {0: 4, 1: 6, 2: 9}[p.system] * 8
 
 
-@pytest.mark.parametrize("steps_per_round", [0, 3])
-def test_served_results_equal_solo(hip, steps_per_round):
+@pytest.mark.parametrize("steps_per_round,policy,cap", [
+    (0, 0, -1), (3, 0, 0), (3, 0, 6000), (0, 1, 6000), (2, 1, -1),
+])
+def test_served_results_equal_solo(hip, steps_per_round, policy, cap):
+    """cap: cells advanced per round (instances beyond it wait resident);
+    policy 0 = arrival order, 1 = least remaining work first."""
     probs = [problem(hip, *r[:6]) for r in REQS]
     U0s = [states.state(r[0], r[3], r[4], seed=r[6]) for r in REQS]
     biggest = max(3 * state_bytes(p) for p in probs)
     budget = 2 * biggest + 4096  # a few instances at a time: queueing and slot reuse
     ids = []
-    with hip.Server(hbm_budget_bytes=budget, max_active=4, steps_per_round=steps_per_round) as srv:
+    with hip.Server(hbm_budget_bytes=budget, max_active=4, steps_per_round=steps_per_round,
+                    max_active_cells=cap, policy=policy) as srv:
         peak_bytes, peak_active = 0, 0
         for k, (p, U0, r) in enumerate(zip(probs, U0s, REQS)):
             ids.append(srv.submit(p, U0, t_end=r[8], max_steps=r[7]))
