@@ -373,9 +373,10 @@ def main():
     ap.add_argument("--check", type=int, default=64, help="batched: instances compared with solo runs")
     ap.add_argument("--requests", type=int, default=64, help="served: number of requests of the trace")
     ap.add_argument("--rate", type=float, default=20.0, help="served: Poisson arrival rate per second")
-    ap.add_argument("--policy", choices=sorted(POLICIES), default="fifo", help="served: admission order")
+    ap.add_argument("--policy", choices=sorted(POLICIES), default="shortest",
+                    help="served: admission and run order (least remaining work first, or arrival order)")
     ap.add_argument("--active-cells", type=int, default=0,
-                    help="served: cap on cells of running instances (0 = fill the device, -1 = none)")
+                    help="served: cells advanced per round (0 = enough to fill the device, -1 = no cap)")
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="run only a plain device copy of MIB MiB (counter calibration)")

@@ -47,6 +47,8 @@ struct Nrm {
 struct Euler {
   static constexpr int NV = 4;
   static constexpr int WAVES_PER_EU = 2;  // register budget of the stage kernel: 256 per lane
+  // no sound speed is shared between the admissibility check and the flux
+  static constexpr bool HAS_ACACHE = false;
   static constexpr int REFX[2] = {1, -1};  // reflective x-wall negates comp 1
   static constexpr int REFY[2] = {2, -1};
   static constexpr bool HAS_NCP = false;
@@ -144,6 +146,10 @@ struct Euler {
 struct Kapila {
   static constexpr int NV = 6;
   static constexpr int WAVES_PER_EU = 1;
+  // the admissibility check at a node computes the a_sq the primitive flux
+  // at that point needs (kapila_5eqn.c:347-349 and 468-470 are the same
+  // expression): the stage kernel hands it over instead of recomputing
+  static constexpr bool HAS_ACACHE = true;
   static constexpr int REFX[2] = {2, -1};
   static constexpr int REFY[2] = {3, -1};
   static constexpr bool HAS_NCP = true;
@@ -251,6 +257,35 @@ struct Kapila {
     F[5] = 0.0;
     return fabs(un) + sqrt(a_sq);
   }
+  // flux_prim with a_sq supplied (a_in >= 0: the value pad_prim_a computed
+  // at the same point, which also guarantees a_sq >= PRS_FLOOR)
+  template <int DIR>
+  __device__ static double flux_prim_a(const Eos& e, const double* V, double a_in, double* F, Bad& b) {
+    if (a_in < 0.0) return flux_prim<DIR>(e, V, F, b);
+    constexpr double nx = Nrm<DIR>::nx, ny = Nrm<DIR>::ny;
+    double alpha_1 = V[5];
+    double alpha_2 = 1.0 - alpha_1;
+    double rho_1 = V[0];
+    double rho_2 = V[1];
+    double rho = alpha_1 * rho_1 + alpha_2 * rho_2;
+    double u = V[2];
+    double v = V[3];
+    double p = V[4];
+    double G1m1 = e.g1 - 1.0;
+    double G2m1 = e.g2 - 1.0;
+    double en = (p * (alpha_1 * G2m1 + alpha_2 * G1m1) + alpha_1 * e.g1 * e.p1 * G2m1 +
+                 alpha_2 * e.g2 * e.p2 * G1m1) /
+                (G1m1 * G2m1);
+    double E = en + 0.5 * rho * (u * u + v * v);
+    double un = u * nx + v * ny;
+    F[0] = alpha_1 * rho_1 * un;
+    F[1] = alpha_2 * rho_2 * un;
+    F[2] = rho * u * un + p * nx;
+    F[3] = rho * v * un + p * ny;
+    F[4] = (E + p) * un;
+    F[5] = 0.0;
+    return fabs(un) + sqrt(a_in);
+  }
   // kapila_5eqn.c:247-283 (alpha_1 test of :278 kept)
   __device__ static bool pad(const Eos& e, const double* Q) {
     double alpha_1 = Q[5];
@@ -272,6 +307,23 @@ struct Kapila {
     if (rho < RHO_FLOOR) return false;
     if (a_sq < PRS_FLOOR) return false;
     if (alpha_1 < 1.0 || alpha_1 > 1.0) return false;
+    return true;
+  }
+  // pad_prim that also returns its a_sq (for flux_prim_a)
+  __device__ static bool pad_prim_a(const Eos& e, const double* V, double& a_out) {
+    double alpha_1 = V[5];
+    double alpha_2 = 1.0 - alpha_1;
+    double rho_1 = V[0];
+    double rho_2 = V[1];
+    double rho = alpha_1 * rho_1 + alpha_2 * rho_2;
+    double p = V[4];
+    double a1_sq = e.g1 * (p + e.p1) / rho_1;
+    double a2_sq = e.g2 * (p + e.p2) / rho_2;
+    double a_sq = (rho_1 * a1_sq * rho_2 * a2_sq) / (rho * (alpha_2 * rho_1 * a1_sq + alpha_1 * rho_2 * a2_sq));
+    a_out = a_sq;
+    if (rho < RHO_FLOOR) return false;
+    if (a_sq < PRS_FLOOR) return false;
+    if (alpha_1 < 0.0 || alpha_1 > 1.0) return false;
     return true;
   }
   // kapila_5eqn.c:450-479
@@ -364,6 +416,8 @@ struct Kapila {
 struct BN {
   static constexpr int NV = 9;
   static constexpr int WAVES_PER_EU = 1;
+  // no sound speed is shared between the admissibility check and the flux
+  static constexpr bool HAS_ACACHE = false;
   static constexpr int REFX[2] = {1, 5};
   static constexpr int REFY[2] = {2, 6};
   static constexpr bool HAS_NCP = true;
@@ -565,9 +619,11 @@ struct BN {
 
 // Rusanov flux + path-conservative jump (riemann.c:52-105, 112-140).
 // Returns max(s_L, s_R) (PetscMax).
+// aL, aR: a_sq of the two states when the system shares it with the
+// admissibility check (S::HAS_ACACHE), negative otherwise.
 template <class S, bool PRIM, int DIR>
 __device__ __forceinline__ double riemann(const Eos& e, const double* L, const double* R, double* F,
-                                          double* D, Bad& b) {
+                                          double* D, Bad& b, double aL = -1.0, double aR = -1.0) {
   constexpr int NV = S::NV;
   double FL[NV], FR[NV];
   double s;
@@ -575,8 +631,14 @@ __device__ __forceinline__ double riemann(const Eos& e, const double* L, const d
     double QL[NV], QR[NV];
     S::prim2cons(e, L, QL);
     S::prim2cons(e, R, QR);
-    double sl = S::template flux_prim<DIR>(e, L, FL, b);
-    double sr = S::template flux_prim<DIR>(e, R, FR, b);
+    double sl, sr;
+    if constexpr (S::HAS_ACACHE) {
+      sl = S::template flux_prim_a<DIR>(e, L, aL, FL, b);
+      sr = S::template flux_prim_a<DIR>(e, R, aR, FR, b);
+    } else {
+      sl = S::template flux_prim<DIR>(e, L, FL, b);
+      sr = S::template flux_prim<DIR>(e, R, FR, b);
+    }
     s = pmax(sl, sr);
 #pragma unroll
     for (int c = 0; c < NV; ++c) F[c] = 0.5 * (FR[c] + FL[c] - s * (QR[c] - QL[c]));

@@ -249,6 +249,7 @@ struct Tile {
 
     // ---- carried state -------------------------------------------------
     double topPrev[NGF][NV];
+    double topA[NGF];  // a_sq handed over with topPrev (negative: none)
     double Gprev[NV], Eprev[NV], xpartPrev[NV], ncpPrev[NV];
 #pragma unroll
     for (int c = 0; c < NV; ++c) {
@@ -259,6 +260,8 @@ struct Tile {
 #pragma unroll
       for (int q = 0; q < NGF; ++q) topPrev[q][c] = 0.0;
     }
+#pragma unroll
+    for (int q = 0; q < NGF; ++q) topA[q] = -1.0;
     double smax = 0.0;
     unsigned long long err = ~0ull;
     const int seq = (mode == HYPE_MODE_STAGE) ? L.stage + 1 : 0;
@@ -307,10 +310,18 @@ struct Tile {
       for (int p = 0; p < NNODE; ++p)
 #pragma unroll
         for (int c = 0; c < NV; ++c) fv[p][c] = at_node(co, c, p);
-      // admissibility at the nodes (rhs_function_prim.c:393-404)
+      // admissibility at the nodes (rhs_function_prim.c:393-404); with
+      // ACACHE the check's a_sq at each node goes on to the flux there
+      constexpr bool ACACHE = PRIM && S::HAS_ACACHE && N == 2;
+      double an[ACACHE ? NNODE : 1];
       bool pad = true;
 #pragma unroll
-      for (int p = 0; p < NNODE; ++p) pad = pad && (PRIM ? S::pad_prim(eos, fv[p]) : S::pad(eos, fv[p]));
+      for (int p = 0; p < NNODE; ++p) {
+        if constexpr (ACACHE)
+          pad = S::pad_prim_a(eos, fv[p], an[p]) && pad;
+        else
+          pad = pad && (PRIM ? S::pad_prim(eos, fv[p]) : S::pad(eos, fv[p]));
+      }
       if (!pad) {  // rhs_function_prim.c:407-421 / rhs_function.c:409-421
 #pragma unroll
         for (int c = 0; c < NV; ++c) {
@@ -327,6 +338,10 @@ struct Tile {
         for (int p = 0; p < NNODE; ++p)
 #pragma unroll
           for (int c = 0; c < NV; ++c) fv[p][c] = at_node(co, c, p);
+        if constexpr (ACACHE) {
+#pragma unroll
+          for (int p = 0; p < NNODE; ++p) an[p] = -1.0;  // new values: the flux computes a_sq itself
+        }
       }
       if (L.fallback && ownedRow && owned) L.fallback[(size_t)r * nx + col] = (unsigned char)(!pad);
 
@@ -374,9 +389,14 @@ struct Tile {
             vl[c] = __shfl_up(fv[Qd::R(q)][c], 1, HYPE_WAVE);
             vr[c] = fv[Qd::L(q)][c];
           }
+          double al = -1.0, ar = -1.0;
+          if constexpr (ACACHE) {
+            al = __shfl_up(an[Qd::R(q)], 1, HYPE_WAVE);
+            ar = an[Qd::L(q)];
+          }
           if (ownedRow) {
             Bad b{0, 0.0};
-            const double s = riemann<S, PRIM, 0>(eos, vl, vr, Fq, Dq, b);
+            const double s = riemann<S, PRIM, 0>(eos, vl, vr, Fq, Dq, b, al, ar);
             if (xlane) {
               if (s > smax) smax = s;
               if (b.kind) {
@@ -412,7 +432,12 @@ struct Tile {
         for (int q = 0; q < NGF; ++q) {
           double Fq[NV], Dq[NV];
           Bad b{0, 0.0};
-          const double s = riemann<S, PRIM, 1>(eos, topPrev[q], fv[Qd::B(q)], Fq, Dq, b);
+          double ab = -1.0, at = -1.0;
+          if constexpr (ACACHE) {
+            ab = topA[q];
+            at = an[Qd::B(q)];
+          }
+          const double s = riemann<S, PRIM, 1>(eos, topPrev[q], fv[Qd::B(q)], Fq, Dq, b, ab, at);
           if (owned) {
             if (s > smax) smax = s;
             if (b.kind) {
@@ -466,6 +491,10 @@ struct Tile {
         ncpPrev[c] = ncp[c];
 #pragma unroll
         for (int q = 0; q < NGF; ++q) topPrev[q][c] = fv[Qd::T(q)][c];
+      }
+      if constexpr (ACACHE) {
+#pragma unroll
+        for (int q = 0; q < NGF; ++q) topA[q] = an[Qd::T(q)];
       }
 
       __syncthreads();  // the new row is visible to the next iteration

@@ -132,3 +132,42 @@ def test_rhs_inadmissible_reports_first_face(hip, system, prim):
     assert msg.startswith(KIND_TEXT[ref["err"].kind])
     assert (x, y) == (ref["err"].x, ref["err"].y)
     assert np.array_equal(R, ref["rhs"], equal_nan=True)
+
+
+def _rhs_pair(hip, system, prim, U, bc=(T, T, T, T)):
+    ny, nx = U.shape[:2]
+    eos = states.EOS[system]
+    dom = (0.0, 1.0, 0.0, ny / nx)
+    p = hip.make_problem(system, nx, ny, order=2, prim=prim, domain=dom, cfl=0.5, bc=bc, **eos)
+    op = orc.problem(system, nx, ny, N=2, prim=prim, domain=dom, cfl=0.5, bc=bc,
+                     g1=eos["gamma1"], g2=eos["gamma2"], p1=eos["pi1"], p2=eos["pi2"])
+    ref = orc.rhs(op, U)
+    with hip.Context(p) as ctx:
+        R, dt, rc = ctx.rhs(U, allow_inadmissible=True)
+    assert rc == ref["rc"]
+    assert np.array_equal(R, ref["rhs"]) and dt == ref["dt"]
+
+
+@pytest.mark.parametrize("prim", [True, False])
+def test_weno_division_range_fallback(hip, prim):
+    """The WENO weights use a shared-reciprocal division that is exact only
+    for divisors up to 2^380; larger smoothness indicators (pressure jumps of
+    1e20 here, IS ~ 1e40) take the plain-division path lane by lane.  Both
+    paths, mixed within one wave, must match the oracle bit for bit."""
+    nx, ny = 70, 24
+    g = states.EOS[0]["gamma1"]
+    X = (np.arange(nx) + 0.5) / nx
+    rho = np.ones((ny, nx)) + 0.1 * np.sin(7 * X)[None, :]
+    u = np.zeros((ny, nx))
+    v = 0.01 * np.cos(5 * X)[None, :] * np.ones((ny, 1))
+    pr = np.where(X[None, :] < 0.5, 1.0e20, 1.0) * (1.0 + 0.1 * np.cos(3 * X)[None, :]) * np.ones((ny, 1))
+    U = states.euler_cons(rho, u, v, pr, g)
+    _rhs_pair(hip, 0, prim, U)
+
+
+def test_small_magnitudes_keep_fast_path_exact(hip):
+    """State scaled by 1e-6 (still admissible): every smoothness indicator is
+    far below the 1e-12 floor, so the weights sit near their linear values."""
+    U = states.state(0, 64, 32, seed=5)
+    U = U * 1.0e-6
+    _rhs_pair(hip, 0, True, U)
