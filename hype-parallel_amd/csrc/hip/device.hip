@@ -209,6 +209,6 @@ int hyped_event_elapsed_ms(void* a, void* b, float* ms) {
   return HYPE_OK;
 }
 
-int hyped_num_stages(int order) { return order == 0 ? 1 : (order == 1 ? 2 : 4); }
+int hyped_num_stages(int order) { return order == 0 ? 1 : (order == 1 ? 2 : (order == 2 ? 4 : 10)); }
 
 }  // extern "C"

@@ -192,6 +192,96 @@ __device__ __forceinline__ bool weno3_fast(const double* U_x, const double* U_y,
   return ok;
 }
 
+// reconstruction.c:223-357: CWENO with one fourth-degree central candidate
+// and five third-degree candidates (gamma_hi = gamma_lo = 0.85)
+__device__ __forceinline__ void weno4(const double* U_x, const double* U_y, const double* U_xy, double* coeffs) {
+  constexpr double r1_6 = 1. / 6., r13_3 = 13. / 3., r7_6 = 7. / 6., r11_120 = 11. / 120., r1_12 = 1. / 12.,
+                   r41_60 = 41. / 60.;
+  const double gammaHi = 0.85, gammaLo = 0.85;
+  double u_0 = U_xy[0];
+  double u_ip1 = U_x[3], u_jp1 = U_y[3], u_ip1jp1 = U_xy[1];
+  double u_im1 = U_x[1], u_jm1 = U_y[1], u_ip1jm1 = U_xy[2];
+  double u_ip2 = U_x[4], u_jp2 = U_y[4], u_im1jp1 = U_xy[3];
+  double u_im2 = U_x[0], u_jm2 = U_y[0], u_im1jm1 = U_xy[4];
+  double gR3[5], xR3[5], yR3[5], xxR3[5], yyR3[5], xyR3[5], wR3[5];
+  const double gR4 = gammaHi;
+  gR3[0] = (1.0 - gammaHi) * gammaLo;
+#pragma unroll
+  for (int i = 1; i < 5; ++i) gR3[i] = 0.25 * (1 - gammaHi) * (1 - gammaLo);
+
+  const double xR4 = r41_60 * (-u_im1 + u_ip1) + r11_120 * (u_im2 - u_ip2);
+  const double yR4 = r41_60 * (-u_jm1 + u_jp1) + r11_120 * (u_jm2 - u_jp2);
+  const double xxR4 = -u_0 + 0.5 * (u_im1 + u_ip1);
+  const double yyR4 = -u_0 + 0.5 * (u_jm1 + u_jp1);
+  const double xyR4 = 0.25 * (u_im1jm1 - u_im1jp1 - u_ip1jm1 + u_ip1jp1);
+  const double xxxR4 = r1_6 * (u_im1 - u_ip1) + r1_12 * (u_ip2 - u_im2);
+  const double yyyR4 = r1_6 * (u_jm1 - u_jp1) + r1_12 * (u_jp2 - u_jm2);
+  const double xxyR4 = 0.25 * (-u_im1jm1 + u_im1jp1 - u_ip1jm1 + u_ip1jp1) + 0.5 * (u_jm1 - u_jp1);
+  const double xyyR4 = 0.5 * (u_im1 - u_ip1) + 0.25 * (u_ip1jp1 - u_im1jm1 - u_im1jp1 + u_ip1jm1);
+  const double IS_R4 = (xR4 + 0.1 * xxxR4) * (xR4 + 0.1 * xxxR4) + r13_3 * (xxR4 * xxR4 + yyR4 * yyR4) +
+                       (yR4 + 0.1 * yyyR4) * (yR4 + 0.1 * yyyR4) + 39.05 * (xxxR4 * xxxR4 + yyyR4 * yyyR4) +
+                       r7_6 * xyR4 * xyR4 + 4.7 * (xxyR4 * xxyR4 + xyyR4 * xyyR4);
+  double wR4 = gR4 / (pow4(IS_R4 + SMALL_NUM));
+  double sum = wR4;
+
+  xR3[0] = 0.5 * (-u_im1 + u_ip1);
+  yR3[0] = 0.5 * (-u_jm1 + u_jp1);
+  xxR3[0] = -u_0 + 0.5 * (u_im1 + u_ip1);
+  yyR3[0] = -u_0 + 0.5 * (u_jm1 + u_jp1);
+  xyR3[0] = 0.25 * (u_im1jm1 - u_im1jp1 - u_ip1jm1 + u_ip1jp1);
+
+  xR3[1] = 1.5 * u_0 - 2.0 * u_im1 + 0.5 * u_im2;
+  yR3[1] = 0.5 * (-u_jm1 + u_jp1);
+  xxR3[1] = 0.5 * u_0 - u_im1 + 0.5 * u_im2;
+  yyR3[1] = -u_0 + 0.5 * u_jm1 + 0.5 * u_jp1;
+  xyR3[1] = 0.5 * (u_im1jm1 - u_im1jp1 - u_jm1 + u_jp1);
+
+  xR3[2] = -1.5 * u_0 + 2.0 * u_ip1 - 0.5 * u_ip2;
+  yR3[2] = 0.5 * (-u_jm1 + u_jp1);
+  xxR3[2] = 0.5 * u_0 - u_ip1 + 0.5 * u_ip2;
+  yyR3[2] = -u_0 + 0.5 * u_jm1 + 0.5 * u_jp1;
+  xyR3[2] = 0.5 * (-u_ip1jm1 + u_ip1jp1 + u_jm1 - u_jp1);
+
+  xR3[3] = 0.5 * (-u_im1 + u_ip1);
+  yR3[3] = 1.5 * u_0 - 2.0 * u_jm1 + 0.5 * u_jm2;
+  xxR3[3] = -u_0 + 0.5 * u_im1 + 0.5 * u_ip1;
+  yyR3[3] = 0.5 * u_0 - u_jm1 + 0.5 * u_jm2;
+  xyR3[3] = 0.5 * (-u_im1 + u_im1jm1 + u_ip1 - u_ip1jm1);
+
+  xR3[4] = 0.5 * (-u_im1 + u_ip1);
+  yR3[4] = -1.5 * u_0 + 2.0 * u_jp1 - 0.5 * u_jp2;
+  xxR3[4] = -u_0 + 0.5 * u_im1 + 0.5 * u_ip1;
+  yyR3[4] = 0.5 * u_0 - u_jp1 + 0.5 * u_jp2;
+  xyR3[4] = 0.5 * (u_im1 - u_im1jp1 - u_ip1 + u_ip1jp1);
+
+#pragma unroll
+  for (int i = 0; i < 5; ++i) {
+    const double IS = xR3[i] * xR3[i] + yR3[i] * yR3[i] + r13_3 * (xxR3[i] * xxR3[i] + yyR3[i] * yyR3[i]) +
+                      r7_6 * xyR3[i] * xyR3[i];
+    wR3[i] = gR3[i] / (pow4(IS + SMALL_NUM));
+    sum = sum + wR3[i];
+  }
+  wR4 = wR4 / sum;
+#pragma unroll
+  for (int i = 0; i < 5; ++i) wR3[i] = wR3[i] / sum;
+  const double wt_ratio = wR4 / gR4;
+
+  auto coef = [&](double R4, const double* R3) {
+    return wt_ratio * (R4 - gR3[0] * R3[0] - gR3[1] * R3[1] - gR3[2] * R3[2] - gR3[3] * R3[3] - gR3[4] * R3[4]) +
+           wR3[0] * R3[0] + wR3[1] * R3[1] + wR3[2] * R3[2] + wR3[3] * R3[3] + wR3[4] * R3[4];
+  };
+  coeffs[0] = u_0;
+  coeffs[1] = coef(xR4, xR3);
+  coeffs[2] = coef(yR4, yR3);
+  coeffs[3] = coef(xxR4, xxR3);
+  coeffs[4] = coef(yyR4, yyR3);
+  coeffs[5] = coef(xyR4, xyR3);
+  coeffs[6] = wt_ratio * xxxR4;
+  coeffs[7] = wt_ratio * yyyR4;
+  coeffs[8] = wt_ratio * xxyR4;
+  coeffs[9] = wt_ratio * xyyR4;
+}
+
 // Stencil of one component: x[0..4] = u(i-2..i+2, j), y[0..4] = u(i, j-2..j+2),
 // d[0..4] = u(i,j), u(i+1,j+1), u(i+1,j-1), u(i-1,j+1), u(i-1,j-1)
 // (rhs_function_prim.c:332-348).
@@ -206,6 +296,8 @@ __device__ __forceinline__ void reconstruct(const double* U_x, const double* U_y
     coeffs[0] = u_0;
     coeffs[1] = minmod(U_x[3] - u_0, u_0 - U_x[1]);
     coeffs[2] = minmod(U_y[3] - u_0, u_0 - U_y[1]);
+  } else if constexpr (N == 3) {
+    weno4(U_x, U_y, U_xy, coeffs);
   } else {
     if (__builtin_expect(weno3_fast(U_x, U_y, U_xy, coeffs), 1)) return;
     // reconstruction.c:144-221

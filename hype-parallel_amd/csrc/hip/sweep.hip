@@ -50,6 +50,12 @@ __device__ __forceinline__ StageDesc stage_desc(int order, int s) {
     if (s == 0) return StageDesc{0, 1, 0, 0, 0, 0.0, 0.0, 2 - 1.};
     return StageDesc{1, 0, 1, 0, 1, (2 - 1.) / 2, 1. / 2, 2.0};
   }
+  if (order == 3) {  // TSSSPRK104: w0 in A/B (ping-pong), w1 built in U at stage 4
+    if (s < 4) return StageDesc{(s & 1) ? 2 : (s == 0 ? 0 : 1), (s & 1) ? 1 : 2, 0, 0, 0, 0.0, 0.0, 6.0};
+    if (s == 4) return StageDesc{2, 0, 0, 0, 2, 1. / 25, 9. / 25, 6.0};
+    if (s < 9) return StageDesc{(s & 1) ? 1 : 2, (s & 1) ? 2 : 1, 0, 0, 0, 0.0, 0.0, 6.0};
+    return StageDesc{1, 0, 1, 0, 1, 3. / 5, 1.0, 10.0};
+  }
   // TSSSPRKS3, s = 4 (n = 2, r = 2)
   if (s == 0) return StageDesc{0, 1, 0, 0, 0, 0.0, 0.0, 2.0};
   if (s == 1) return StageDesc{1, 2, 0, 0, 0, 0.0, 0.0, 2.0};
@@ -57,7 +63,9 @@ __device__ __forceinline__ StageDesc stage_desc(int order, int s) {
   return StageDesc{1, 0, 0, 0, 0, 0.0, 0.0, 2.0};
 }
 
-__device__ __forceinline__ int num_stages_dev(int order) { return order == 0 ? 1 : (order == 1 ? 2 : 4); }
+__device__ __forceinline__ int num_stages_dev(int order) {
+  return order == 0 ? 1 : (order == 1 ? 2 : (order == 2 ? 4 : 10));
+}
 
 // packed error word (layout in hype_internal.h)
 __device__ __forceinline__ unsigned long long pack_err(int seq, long key, int kind, double val) {
@@ -473,6 +481,18 @@ struct Tile {
           if (sd.op == 0) {
 #pragma unroll
             for (int c = 0; c < NV; ++c) out[idx + c] = Uin[idx + c] + cdt * rhs[c];
+          } else if (sd.op == 2) {
+            // RK104 after its fifth stage: w0 += dt/6 F (VecAXPY); w1 =
+            // 1/25 U + 9/25 w0 (VecAXPBYPCZ, c = 0) into U; w0 = 15 w1 - 5 w0
+            // (VecAXPBY) into A
+            double* A = I->buf[1];
+#pragma unroll
+            for (int c = 0; c < NV; ++c) {
+              const double w0 = Uin[idx + c] + cdt * rhs[c];
+              const double w1 = sd.a * out[idx + c] + sd.g * w0;
+              out[idx + c] = w1;
+              A[idx + c] = 15.0 * w1 + -5.0 * w0;
+            }
           } else {
             const double* X = I->buf[sd.xb];
             const double* Z = I->buf[sd.zb];
@@ -560,6 +580,7 @@ static hipError_t launch_sys(const hype_launch& L, hipStream_t s) {
   switch (L.v.order) {
     case 0: return p ? launch_variant<S, true, 0>(L, s) : launch_variant<S, false, 0>(L, s);
     case 1: return p ? launch_variant<S, true, 1>(L, s) : launch_variant<S, false, 1>(L, s);
+    case 3: return p ? launch_variant<S, true, 3>(L, s) : launch_variant<S, false, 3>(L, s);
     default: return p ? launch_variant<S, true, 2>(L, s) : launch_variant<S, false, 2>(L, s);
   }
 }
@@ -578,6 +599,7 @@ static int occupancy_sys(const hype_variant& v) {
   switch (v.order) {
     case 0: return p ? occupancy_variant<S, true, 0>() : occupancy_variant<S, false, 0>();
     case 1: return p ? occupancy_variant<S, true, 1>() : occupancy_variant<S, false, 1>();
+    case 3: return p ? occupancy_variant<S, true, 3>() : occupancy_variant<S, false, 3>();
     default: return p ? occupancy_variant<S, true, 2>() : occupancy_variant<S, false, 2>();
   }
 }

@@ -20,7 +20,7 @@ int hype_check_problem(const hype_problem *p, int *nvar)
     if (!p) return HYPE_E_INVALID;
     nv = hype_nvar(p->system);
     if (nv == 0) return HYPE_E_INVALID;
-    if (p->order < 0 || p->order > 2) return HYPE_E_UNSUPPORTED;
+    if (p->order < 0 || p->order > 3) return HYPE_E_UNSUPPORTED;
     if (p->nx < 3 || p->ny < 3) return HYPE_E_INVALID;
     /* face keys of the error word hold 28 bits */
     if ((double)(p->nx + 1) * p->ny * 2.0 * 2.0 >= (double)(1 << 28)) return HYPE_E_UNSUPPORTED;
@@ -138,4 +138,22 @@ void hype_fill_inst(hype_inst *I, const hype_problem *p, int nsy)
     I->p1 = p->pi1;
     I->p2 = p->pi2;
     I->quirks = p->quirks;
+}
+
+/* Cell-centre coordinates.  HYPE_COORD_EXACT: x_min + h/2 + i h.
+ * HYPE_COORD_DMDA: the coordinates DMDASetUniformCoordinates(x_min + h/2,
+ * x_max + h/2, ...) gives the reference (hype.c:132-135): spacing h on a
+ * periodic axis, span/(n - 1) otherwise. */
+double hype_cell_coord(const hype_problem *p, int axis, int i, int coord_mode)
+{
+    const double h = (p->x_max - p->x_min) / (double)p->nx; /* hype.c:57 */
+    double d = h;
+    if (axis == 0) {
+        if (coord_mode == HYPE_COORD_DMDA && p->bc[0] != HYPE_BC_PERIODIC && p->bc[1] != HYPE_BC_PERIODIC)
+            d = (p->x_max - p->x_min) / (double)(p->nx - 1);
+        return p->x_min + 0.5 * h + (double)i * d;
+    }
+    if (coord_mode == HYPE_COORD_DMDA && p->bc[2] != HYPE_BC_PERIODIC && p->bc[3] != HYPE_BC_PERIODIC)
+        d = ((p->y_max + 0.5 * h) - (p->y_min + 0.5 * h)) / (double)(p->ny - 1);
+    return p->y_min + 0.5 * h + (double)i * d;
 }

@@ -31,6 +31,9 @@ void hype_decode_error(unsigned long long word, int nx, int ny, int ngf,
                        double x_min, double y_min, double h, char *msg,
                        size_t cap, double *x, double *y);
 
+/* x (axis 0) or y (axis 1) coordinate of cell i (HYPE_COORD_* mode) */
+double hype_cell_coord(const hype_problem *p, int axis, int i, int coord_mode);
+
 /* fill the device-independent fields of an instance descriptor */
 void hype_fill_inst(hype_inst *I, const hype_problem *p, int nsy);
 

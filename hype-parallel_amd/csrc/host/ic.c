@@ -175,7 +175,7 @@ int hype_initial_state(const hype_problem *p, int kind, const double *params, in
     static const int need[] = {0, 18, 10, 13, 0};
     ic_ctx C;
     int nv = 0, i, j, l, m, c;
-    double h, hx0, hy0;
+    double h;
     HYPE_TRY(hype_check_problem(p, &nv));
     if (!U || kind < 0 || kind > HYPE_IC_SMOOTH_VORTEX) return HYPE_E_INVALID;
     if (kind == HYPE_IC_UNIFORM && nparams < nv) return HYPE_E_INVALID;
@@ -195,19 +195,11 @@ int hype_initial_state(const hype_problem *p, int kind, const double *params, in
         for (k = 0; k < 8; ++k) C.modes[0][k] = 2.0 * M_PI * hype_rng_uniform(&s);
     }
     h = (p->x_max - p->x_min) / (double)p->nx; /* hype.c:57 */
-    hx0 = h;
-    hy0 = h;
-    if (coord_mode == HYPE_COORD_DMDA) {
-        if (p->bc[0] != HYPE_BC_PERIODIC && p->bc[1] != HYPE_BC_PERIODIC)
-            hx0 = (p->x_max - p->x_min) / (double)(p->nx - 1);
-        if (p->bc[2] != HYPE_BC_PERIODIC && p->bc[3] != HYPE_BC_PERIODIC)
-            hy0 = ((p->y_max + 0.5 * h) - (p->y_min + 0.5 * h)) / (double)(p->ny - 1);
-    }
     for (j = 0; j < p->ny; ++j)
         for (i = 0; i < p->nx; ++i) {
             double integral[HYPE_MAXVAR], V[HYPE_MAXVAR], Q[HYPE_MAXVAR];
-            double xc = p->x_min + 0.5 * h + (double)i * hx0;
-            double yc = p->y_min + 0.5 * h + (double)j * hy0;
+            double xc = hype_cell_coord(p, 0, i, coord_mode);
+            double yc = hype_cell_coord(p, 1, j, coord_mode);
             for (c = 0; c < nv; ++c) integral[c] = 0.0;
             for (l = 0; l < 5; ++l)
                 for (m = 0; m < 5; ++m) {

@@ -180,6 +180,9 @@ def lib():
     L.hype_server_now.restype = C.c_double
     L.hype_server_bytes_in_use.argtypes = [vp]
     L.hype_server_bytes_in_use.restype = C.c_size_t
+    L.hype_write_vec_binary.argtypes = [C.c_char_p, dp, C.c_long, C.c_int]
+    L.hype_read_vec_binary.argtypes = [C.c_char_p, dp, C.c_long]
+    L.hype_write_vts.argtypes = [C.c_char_p, P(Problem), dp, C.c_int]
     _lib = L
     return L
 
@@ -196,7 +199,7 @@ def exported_symbols():
         "hype_batch_sync", "hype_batch_last_error", "hype_batch_tiling", "hype_batch_set_rows",
         "hype_batch_autotune", "hype_server_create", "hype_server_destroy", "hype_server_submit",
         "hype_server_poll", "hype_server_drain", "hype_server_info", "hype_server_now",
-        "hype_server_bytes_in_use",
+        "hype_server_bytes_in_use", "hype_write_vec_binary", "hype_read_vec_binary", "hype_write_vts",
     ]
 
 
@@ -216,6 +219,31 @@ def initial_state(problem, kind, params=(), seed=0, coord=COORD_EXACT):
     if rc != OK:
         raise HypeError(rc, "hype_initial_state")
     return U
+
+
+def write_restart(path, U, block_size=None):
+    """State U in PETSc's binary Vec format (restart1.bin of the reference)."""
+    U = np.ascontiguousarray(U, dtype=np.float64)
+    bs = U.shape[-1] if block_size is None else block_size
+    rc = lib().hype_write_vec_binary(os.fsencode(path), _dptr(U), U.size, bs)
+    if rc != OK:
+        raise HypeError(rc, f"hype_write_vec_binary({path})")
+
+
+def read_restart(path, shape):
+    U = np.empty(shape, dtype=np.float64)
+    rc = lib().hype_read_vec_binary(os.fsencode(path), _dptr(U), U.size)
+    if rc != OK:
+        raise HypeError(rc, f"hype_read_vec_binary({path})")
+    return U
+
+
+def write_vts(path, problem, W, coord=COORD_DMDA):
+    """Primitive state W as a VTK XML structured grid (sol-*.vts of the reference)."""
+    W = np.ascontiguousarray(W, dtype=np.float64)
+    rc = lib().hype_write_vts(os.fsencode(path), C.byref(problem), _dptr(W), coord)
+    if rc != OK:
+        raise HypeError(rc, f"hype_write_vts({path})")
 
 
 def rankine_hugoniot(gamma, rho0, p0, mach):

@@ -183,6 +183,23 @@ int hype_initial_state(const hype_problem *p, int kind, const double *params,
 void hype_rankine_hugoniot(double gamma, double rho0, double p0, double M,
                            double *rho1, double *u1, double *p1);
 
+/* ---- on-disk formats -------------------------------------------------- */
+/* Restart file in PETSc's binary Vec format (VecView/VecLoad on a binary
+ * viewer, post_process.c:47-63, hype.c:316-330): big-endian int32 class id
+ * 1211214, int32 length n, n big-endian doubles; block_size > 1 also writes
+ * <path>.info with "-vecload_block_size <block_size>".  The state layout is
+ * the DMDA natural ordering, i.e. [j][i][c] on one rank. */
+int hype_write_vec_binary(const char *path, const double *U, long n, int block_size);
+/* HYPE_E_NOTFOUND if the file is missing, HYPE_E_INVALID if its header does
+ * not describe a Vec of length n */
+int hype_read_vec_binary(const char *path, double *U, long n);
+
+/* Primitive state W (as hype_cons2prim returns it) as a VTK XML structured
+ * grid, one scalar array per component, points at the cell centres
+ * (coord_mode HYPE_COORD_*): the sol-*.vts snapshots of MonitorFunction
+ * (post_process.c:29-41) that plot.py reads. */
+int hype_write_vts(const char *path, const hype_problem *p, const double *W, int coord_mode);
+
 /* ---- tiling (used by the autotuner and the tests) ------------------- */
 /* Rows per tile of the line-sweep kernel and persistent grid size (0 = one
  * workgroup per tile).  Results do not depend on either. */

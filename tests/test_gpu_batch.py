@@ -71,12 +71,12 @@ def test_mixed_batch_equals_solo(hip, tile_rows):
 
 def test_batch_matches_oracle(hip):
     cases = [(1, True, 2, 48, 30, (T, T, R, R), 21), (2, False, 2, 40, 24, (P, P, R, R), 22),
-             (0, True, 1, 36, 36, (R, R, R, R), 23)]
+             (0, True, 1, 36, 36, (R, R, R, R), 23), (1, True, 3, 44, 28, (T, T, R, R), 24)]
     probs, U0s = [], []
     for s, pr, o, nx, ny, bc, seed in cases:
         probs.append(problem(hip, s, pr, o, nx, ny, bc))
         U0s.append(states.state(s, nx, ny, seed=seed))
-    steps = np.array([7, 9, 11], dtype=np.int32)
+    steps = np.array([7, 9, 11, 5], dtype=np.int32)
     out, st, _ = run_batch(hip, probs, U0s, steps)
     for i, (s, pr, o, nx, ny, bc, seed) in enumerate(cases):
         eos = states.EOS[s]

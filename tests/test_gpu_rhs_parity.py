@@ -62,7 +62,7 @@ def check_rhs(hip, system, prim, order, bc, nx, ny, seed, hrows=None, grid=0, qu
 
 @pytest.mark.parametrize("system", [0, 1, 2])
 @pytest.mark.parametrize("prim", [True, False])
-@pytest.mark.parametrize("order", [0, 1, 2])
+@pytest.mark.parametrize("order", [0, 1, 2, 3])
 def test_rhs_systems_paths_orders(hip, system, prim, order):
     quirks = orc.QUIRK_BN_P2_IS_PHI if system == 2 else 0
     check_rhs(hip, system, prim, order, BCS["refl_tb"], 75, 40, seed=1, quirks=quirks)
@@ -86,11 +86,11 @@ def test_rhs_ragged_sizes(hip, nx, ny):
 
 @pytest.mark.parametrize("system", [0, 1, 2])
 @pytest.mark.parametrize("prim", [True, False])
-@pytest.mark.parametrize("order", [1, 2])
+@pytest.mark.parametrize("order", [1, 2, 3])
 def test_rhs_fallback_exercised(hip, system, prim, order):
     quirks = orc.QUIRK_BN_P2_IS_PHI if system == 2 else 0
     mask = check_rhs(hip, system, prim, order, BCS["refl_tb"], 96, 80, seed=5, quirks=quirks, rough=True)
-    if order == 2:  # minmod slopes (N=1) never leave the admissible set
+    if order >= 2:  # minmod slopes (N=1) never leave the admissible set
         assert mask.sum() > 0, "state should trigger the admissibility fallback somewhere"
 
 

@@ -57,7 +57,7 @@ def run_pair(hip, system, prim, order, bc, nx, ny, steps, seed, t_end=float("inf
 
 @pytest.mark.parametrize("system,prim,order", [
     (0, True, 2), (0, False, 2), (1, True, 2), (1, False, 1), (2, True, 2), (2, False, 2), (0, True, 1),
-    (1, True, 1),
+    (1, True, 1), (0, True, 3), (1, True, 3), (2, False, 3),
 ])
 def test_steps_match_oracle(hip, system, prim, order):
     # the primitive Baer-Nunziato flux with p_2 = V[8] (baer_nunziato.c:271)
