@@ -43,7 +43,7 @@ struct StageDesc {
   double a, g, bden;
 };
 
-__device__ __forceinline__ StageDesc stage_desc(int order, int s) {
+__host__ __device__ __forceinline__ StageDesc stage_desc(int order, int s) {
   if (order == 0)  // TSEULER: update = U + dt*F, copied back to U afterwards
     return StageDesc{0, 1, 0, 0, 0, 0.0, 0.0, 1.0};
   if (order == 1) {  // TSSSPRKS2, s = 2
@@ -51,7 +51,9 @@ __device__ __forceinline__ StageDesc stage_desc(int order, int s) {
     return StageDesc{1, 0, 1, 0, 1, (2 - 1.) / 2, 1. / 2, 2.0};
   }
   if (order == 3) {  // TSSSPRK104: w0 in A/B (ping-pong), w1 built in U at stage 4
-    if (s < 4) return StageDesc{(s & 1) ? 2 : (s == 0 ? 0 : 1), (s & 1) ? 1 : 2, 0, 0, 0, 0.0, 0.0, 6.0};
+    // stages 0-3: U -> A -> B -> A -> B; stage 4 reads B and writes U and A;
+    // stages 5-8: A -> B -> A -> B -> A; stage 9 reads A and writes U
+    if (s < 4) return StageDesc{s == 0 ? 0 : ((s & 1) ? 1 : 2), (s & 1) ? 2 : 1, 0, 0, 0, 0.0, 0.0, 6.0};
     if (s == 4) return StageDesc{2, 0, 0, 0, 2, 1. / 25, 9. / 25, 6.0};
     if (s < 9) return StageDesc{(s & 1) ? 1 : 2, (s & 1) ? 2 : 1, 0, 0, 0, 0.0, 0.0, 6.0};
     return StageDesc{1, 0, 1, 0, 1, 3. / 5, 1.0, 10.0};
@@ -63,7 +65,7 @@ __device__ __forceinline__ StageDesc stage_desc(int order, int s) {
   return StageDesc{1, 0, 0, 0, 0, 0.0, 0.0, 2.0};
 }
 
-__device__ __forceinline__ int num_stages_dev(int order) {
+__host__ __device__ __forceinline__ int num_stages_dev(int order) {
   return order == 0 ? 1 : (order == 1 ? 2 : (order == 2 ? 4 : 10));
 }
 
@@ -648,3 +650,19 @@ hipError_t launch_cons2prim(int system, const double* U, double* W, long n, Eos 
 }
 
 }  // namespace hype
+
+// The stage table as the host sees it (tests replay the buffer algebra on
+// the CPU against the oracle's TSSSP steps).
+extern "C" int hyped_stage_table(int order, int s, int* ibuf, double* coef) {
+  if (order < 0 || order > 3 || s < 0 || s >= hype::num_stages_dev(order)) return -1;
+  const hype::StageDesc d = hype::stage_desc(order, s);
+  ibuf[0] = d.in;
+  ibuf[1] = d.out;
+  ibuf[2] = d.xb;
+  ibuf[3] = d.zb;
+  ibuf[4] = d.op;
+  coef[0] = d.a;
+  coef[1] = d.g;
+  coef[2] = d.bden;
+  return hype::num_stages_dev(order);
+}

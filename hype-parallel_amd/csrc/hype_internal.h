@@ -100,6 +100,9 @@ typedef struct hype_launch {
 
 /* number of stages of the scheme for order N */
 int hyped_num_stages(int order);
+/* stage s of the order-N scheme: ibuf = {in, out, xb, zb, op}, coef = {a, g,
+ * bden}; returns the number of stages, or -1 */
+int hyped_stage_table(int order, int s, int *ibuf, double *coef);
 
 /* ---- thin device ABI (implemented in csrc/hip/device.hip) -------------- */
 int hyped_init(void);
