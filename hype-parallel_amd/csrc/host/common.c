@@ -63,20 +63,34 @@ int hype_strips(int nx) { return (nx + HYPE_TILE_W - 1) / HYPE_TILE_W; }
 /* Tile-shape model.  A wave sweeps its strip one row per iteration with all
  * 64 lanes busy whatever the strip width, so a tile of h owned rows costs
  * about (h + 2 + c0) row-iterations (two ring rows, c0 for the 5-row
- * prologue).  Tiles run `slots` at a time (resident waves of the device), so
- * a launch of T tiles takes ceil(T/slots) rounds of the tallest tile.
- * Returns the number of row segments minimising rounds x tile cost. */
-int hype_pick_segments(int nx, int ny, int slots, int extra_tiles)
+ * prologue).  The kernel is bound by FP64 issue: waves that share a SIMD
+ * share its issue slots, gaining only latency hiding (a measured 1.5x for
+ * two waves).  With w = ceil(T / simds) waves per SIMD and `occ` resident
+ * waves per SIMD, a launch of T tiles therefore takes about
+ *   (h + 2 + c0) * w / gain(w)                      if w <= occ,
+ *   (h + 2 + c0) * ceil(w / occ) * occ / gain(occ)   otherwise.
+ * Returns the number of row segments minimising that. */
+static double occ_gain(long w) { return w >= 2 ? 1.5 : 1.0; }
+
+double hype_tile_cost(long tiles, int hmax, int simds, int occ)
 {
     const double c0 = 1.0;
+    const double per = hmax + 2.0 + c0;
+    long w;
+    if (simds < 1) simds = 1;
+    if (occ < 1) occ = 1;
+    w = (tiles + simds - 1) / simds;
+    if (w <= occ) return per * (double)w / occ_gain(w);
+    return per * (double)((w + occ - 1) / occ) * occ / occ_gain(occ);
+}
+
+int hype_pick_segments(int nx, int ny, int simds, int occ, int extra_tiles)
+{
     int nsx = hype_strips(nx), k, best = 1;
     double best_cost = 1e300;
-    if (slots < 1) slots = 1;
     for (k = 1; k <= ny; ++k) {
         long tiles = (long)nsx * k + extra_tiles;
-        long rounds = (tiles + slots - 1) / slots;
-        int hmax = (ny + k - 1) / k;
-        double cost = (double)rounds * (hmax + 2.0 + c0);
+        double cost = hype_tile_cost(tiles, (ny + k - 1) / k, simds, occ);
         if (cost < best_cost - 1e-9) {
             best_cost = cost;
             best = k;

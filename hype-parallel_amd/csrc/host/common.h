@@ -17,9 +17,11 @@ int hype_check_problem(const hype_problem *p, int *nvar);
 /* strips of width <= HYPE_TILE_W covering nx columns */
 int hype_strips(int nx);
 
-/* number of row segments for a grid on a device with `slots` resident
- * waves of the kernel (tile-shape model in common.c) */
-int hype_pick_segments(int nx, int ny, int slots, int extra_tiles);
+/* number of row segments for a grid on a device with `simds` SIMDs and
+ * `occ` resident waves of the kernel per SIMD (tile-shape model in common.c) */
+int hype_pick_segments(int nx, int ny, int simds, int occ, int extra_tiles);
+/* modelled time of a launch of `tiles` tiles at most hmax rows tall */
+double hype_tile_cost(long tiles, int hmax, int simds, int occ);
 
 /* build the tile list of one instance (nsx strips x nsy segments, near-equal
  * splits) into tiles (capacity cap); returns the number of tiles, the first

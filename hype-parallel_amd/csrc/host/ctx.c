@@ -77,7 +77,11 @@ int hype_ctx_create(const hype_problem *p, hype_ctx **out)
         c->slots = hyped_num_cus() * hyped_blocks_per_cu(&v);
         if (c->slots < 1) c->slots = 1024;
     }
-    c->nsy = hype_pick_segments(p->nx, p->ny, c->slots, 0);
+    {
+        int ncu = hyped_num_cus();
+        if (ncu < 1) ncu = 256;
+        c->nsy = hype_pick_segments(p->nx, p->ny, 4 * ncu, (c->slots + 4 * ncu - 1) / (4 * ncu), 0);
+    }
     c->ntiles = hype_strips(p->nx) * c->nsy;
     c->grid = 0;
     if ((rc = hyped_stream_create(&c->stream)) != HYPE_OK) goto fail;

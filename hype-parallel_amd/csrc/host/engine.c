@@ -183,26 +183,30 @@ static int ensure_cap(hype_engine *E, eng_group *G, int n)
     return HYPE_OK;
 }
 
-/* Common tile height for the used slots: the tile-shape model of
- * hype_pick_segments applied to the union of all tiles, per group slots. */
+/* Common tile height for the running slots: the tile-shape model of
+ * hype_pick_segments applied to the union of all tiles, with the smallest
+ * occupancy among the groups. */
 int eng_pick_rows(hype_engine *E)
 {
-    int h, best = 16, slots = 0, g;
+    int h, best = 16, g, simds = 4 * E->ncu, occ = 1 << 30;
     double best_cost = 1e300;
-    for (g = 0; g < E->ng; ++g) slots += E->g[g].grid;
-    if (slots < 1) slots = E->ncu * 4;
+    for (g = 0; g < E->ng; ++g) {
+        int o = E->g[g].grid / simds;
+        if (o < occ) occ = o;
+    }
+    if (occ < 1 || occ == 1 << 30) occ = 1;
     for (h = 1; h <= 128; ++h) {
         long tiles = 0;
-        int s;
+        int s, hmax = 0;
         for (s = 0; s < E->cap; ++s)
             if (E->used[s] && E->run[s]) {
-                int ny = E->P[s].ny, nsy = (ny + h - 1) / h;
+                int ny = E->P[s].ny, nsy = (ny + h - 1) / h, ht = (ny + nsy - 1) / nsy;
                 tiles += (long)hype_strips(E->P[s].nx) * nsy;
+                if (ht > hmax) hmax = ht;
             }
         if (tiles == 0) break;
         {
-            double rounds = (double)((tiles + slots - 1) / slots);
-            double cost = rounds * (h + 2.0 + 1.0);
+            const double cost = hype_tile_cost(tiles, hmax, simds, occ);
             if (cost < best_cost - 1e-9) {
                 best_cost = cost;
                 best = h;
