@@ -69,12 +69,13 @@ __host__ __device__ __forceinline__ int num_stages_dev(int order) {
   return order == 0 ? 1 : (order == 1 ? 2 : (order == 2 ? 4 : 10));
 }
 
-// packed error word (layout in hype_internal.h)
-__device__ __forceinline__ unsigned long long pack_err(int seq, long key, int kind, double val) {
+// packed error word (layout in hype_internal.h): K = face-key bits
+__device__ __forceinline__ unsigned long long pack_err(int seq, long key, int kind, double val, int K) {
   const float f = (float)val;
-  return ((unsigned long long)(seq & 0x7) << HYPE_ERR_SEQ_SHIFT) |
-         ((unsigned long long)(key & 0xFFFFFFF) << HYPE_ERR_FACE_SHIFT) |
-         ((unsigned long long)(kind & 0xF) << HYPE_ERR_KIND_SHIFT) | (unsigned long long)(__float_as_uint(f) >> 3);
+  return ((unsigned long long)(seq & 0xF) << HYPE_ERR_SEQ_SHIFT) |
+         (((unsigned long long)key & ((1ull << K) - 1ull)) << (HYPE_ERR_SEQ_SHIFT - K)) |
+         ((unsigned long long)(kind & 0xF) << (HYPE_ERR_SEQ_SHIFT - K - 4)) |
+         (unsigned long long)(__float_as_uint(f) >> (K - 24));
 }
 
 // map a (possibly ghost) index onto the owned range; toggles flip for a
@@ -411,7 +412,7 @@ struct Tile {
               if (s > smax) smax = s;
               if (b.kind) {
                 const long key = ((long)r * (nx + 1) + col) * NGF + q;
-                const unsigned long long pk = pack_err(seq, key, b.kind, b.val);
+                const unsigned long long pk = pack_err(seq, key, b.kind, b.val, I->keybits);
                 if (pk < err) err = pk;
               }
             }
@@ -452,7 +453,7 @@ struct Tile {
             if (s > smax) smax = s;
             if (b.kind) {
               const long key = (long)ny * (nx + 1) * NGF + ((long)r * nx + col) * NGF + q;
-              const unsigned long long pk = pack_err(seq, key, b.kind, b.val);
+              const unsigned long long pk = pack_err(seq, key, b.kind, b.val, I->keybits);
               if (pk < err) err = pk;
             }
           }

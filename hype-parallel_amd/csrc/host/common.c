@@ -22,8 +22,9 @@ int hype_check_problem(const hype_problem *p, int *nvar)
     if (nv == 0) return HYPE_E_INVALID;
     if (p->order < 0 || p->order > 3) return HYPE_E_UNSUPPORTED;
     if (p->nx < 3 || p->ny < 3) return HYPE_E_INVALID;
-    /* face keys of the error word hold 28 bits */
-    if ((double)(p->nx + 1) * p->ny * 2.0 * 2.0 >= (double)(1 << 28)) return HYPE_E_UNSUPPORTED;
+    /* face keys of the error word hold at most 36 bits (hype_key_bits) */
+    if (((double)(p->nx + 1) * p->ny + (double)p->nx * (p->ny + 1)) * 2.0 >= 68719476736.0)
+        return HYPE_E_UNSUPPORTED;
     if (!(p->x_max > p->x_min) || !(p->y_max > p->y_min)) return HYPE_E_INVALID;
     if (!(p->cfl > 0.0)) return HYPE_E_INVALID;
     for (s = 0; s < 4; ++s) {
@@ -99,6 +100,14 @@ int hype_pick_segments(int nx, int ny, int simds, int occ, int extra_tiles)
     return best;
 }
 
+/* face-key bits of the error word: every face point of the grid
+ * ((nx+1)*ny x-face and nx*(ny+1) y-face points, 2 per face) must fit */
+int hype_key_bits(int nx, int ny)
+{
+    const double faces = ((double)(nx + 1) * ny + (double)nx * (ny + 1)) * 2.0;
+    return faces < 268435456.0 ? HYPE_ERR_KEYBITS_SMALL : HYPE_ERR_KEYBITS_LARGE;
+}
+
 void hype_decode_error(unsigned long long word, int nx, int ny, int ngf,
                        double x_min, double y_min, double h, char *msg,
                        size_t cap, double *x, double *y)
@@ -108,9 +117,10 @@ void hype_decode_error(unsigned long long word, int nx, int ny, int ngf,
         "Negative square speed of sound = %f", "Negative density (solid) = %f",
         "Negative density (gas) = %f", "Negative pressure (solid), p = %f",
         "Negative pressure (gas), p = %f"};
-    long key = (long)((word >> HYPE_ERR_FACE_SHIFT) & 0xFFFFFFFull);
-    int kind = (int)((word >> HYPE_ERR_KIND_SHIFT) & 0xF);
-    unsigned vb = (unsigned)(word & 0x1FFFFFFFull) << 3;
+    const int K = hype_key_bits(nx, ny);
+    long key = (long)((word >> (HYPE_ERR_SEQ_SHIFT - K)) & ((1ull << K) - 1ull));
+    int kind = (int)((word >> (HYPE_ERR_SEQ_SHIFT - K - 4)) & 0xF);
+    unsigned vb = (unsigned)((word & ((1ull << (56 - K)) - 1ull)) << (K - 24));
     float fv;
     long nxf = (long)ny * (nx + 1) * ngf;
     long f;
@@ -152,6 +162,7 @@ void hype_fill_inst(hype_inst *I, const hype_problem *p, int nsy)
     I->p1 = p->pi1;
     I->p2 = p->pi2;
     I->quirks = p->quirks;
+    I->keybits = hype_key_bits(p->nx, p->ny);
 }
 
 /* Cell-centre coordinates.  HYPE_COORD_EXACT: x_min + h/2 + i h.

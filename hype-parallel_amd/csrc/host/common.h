@@ -28,6 +28,9 @@ double hype_tile_cost(long tiles, int hmax, int simds, int occ);
  * flagged as leader, or -1 */
 int hype_make_tiles(int inst, int nx, int ny, int nsx, int nsy, hype_tile *tiles, int cap);
 
+/* face-key bits of the packed error word for an nx x ny grid */
+int hype_key_bits(int nx, int ny);
+
 /* decode the packed error word into a message and face coordinates */
 void hype_decode_error(unsigned long long word, int nx, int ny, int ngf,
                        double x_min, double y_min, double h, char *msg,

@@ -54,7 +54,7 @@ typedef struct hype_inst {
     double x_min, y_min;
     double g1, g2, p1, p2;
     unsigned quirks;
-    int pad1;
+    int keybits;           /* face-key bits of the packed error word (28 or 36) */
 } hype_inst;
 
 /* tile: instance, first owned column and row, owned width (<= HYPE_TILE_W)
@@ -64,13 +64,15 @@ typedef struct hype_tile {
     int inst, x0, y0, w, h, flags;
 } hype_tile;
 
-/* packed error word: call sequence within the step (3 bits: 0 priming or
- * single RHS, 1.. stage), face key (28), check kind (4), the offending value
- * as the top 29 bits of its float32 image.  The smallest word is the lowest
- * face of the earliest failing call, as the reference aborts there. */
-#define HYPE_ERR_SEQ_SHIFT 61
-#define HYPE_ERR_FACE_SHIFT 33
-#define HYPE_ERR_KIND_SHIFT 29
+/* packed error word, most significant first: call sequence within the step
+ * (4 bits: 0 priming or single RHS, 1.. stage; 10 stages at most), face key
+ * (K bits), check kind (4 bits), the offending value as the top 56-K bits of
+ * its float32 image.  K = 28 when the grid has fewer than 2^28 face points
+ * (value mantissa 19 bits), 36 beyond.  The smallest word is the lowest face
+ * of the earliest failing call, as the reference aborts there. */
+#define HYPE_ERR_SEQ_SHIFT 60
+#define HYPE_ERR_KEYBITS_SMALL 28
+#define HYPE_ERR_KEYBITS_LARGE 36
 
 /* kernel modes */
 #define HYPE_MODE_STAGE 0

@@ -171,3 +171,27 @@ def test_small_magnitudes_keep_fast_path_exact(hip):
     U = states.state(0, 64, 32, seed=5)
     U = U * 1.0e-6
     _rhs_pair(hip, 0, True, U)
+
+
+def test_large_grid_reports_first_face(hip):
+    """A grid past 2^28 face points (8200 x 8200) switches the error word to
+    36-bit face keys: the first failing face is still found and located.
+    Uniform Euler state with one negative-pressure cell at (i0, j0): the
+    minmod slopes around it vanish, so the first inadmissible face in the
+    reference's loop order is that cell's left x-face."""
+    n, i0, j0 = 8200, 8001, 8123
+    g = states.EOS[0]["gamma1"]
+    U = np.empty((n, n, 4))
+    U[..., 0], U[..., 1], U[..., 2], U[..., 3] = 1.0, 0.0, 0.0, 1.0 / (g - 1.0)
+    U[j0, i0, 3] = -1.0 / (g - 1.0)  # p = -1
+    p = hip.make_problem(hip.EULER, n, n, order=1, prim=True, domain=(0.0, 1.0, 0.0, 1.0), gamma1=g)
+    with hip.Context(p) as ctx:
+        R, dt, rc = ctx.rhs(U, allow_inadmissible=True)
+        msg, x, y = ctx.last_error()
+    h = 1.0 / n
+    print(f"[large] {msg!r} x={x} y={y}", flush=True)
+    assert rc == hip.E_INADMISSIBLE
+    assert msg.startswith(KIND_TEXT[2]) and "-1.0" in msg
+    assert (x, y) == (0.0 + i0 * h, 0.0 + j0 * h)
+    R[j0 - 1:j0 + 2, i0 - 1:i0 + 2] = 0.0
+    assert not R.any()  # uniform elsewhere: zero RHS
