@@ -385,8 +385,9 @@ def main():
         calibrate_copy(args.calibrate_copy)
         return
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is a design only "
-                                   "(see DESIGN.md), so --gpus must be 1"}))
+        print(json.dumps({"error": "this build runs on one GPU (independent instances would map one per GPU; "
+                                   "a single grid would split into blocks with one halo exchange per RK "
+                                   "stage, not built here), so --gpus must be 1"}))
         sys.exit(2)
     if args.config == "batched":
         run_batched(args)

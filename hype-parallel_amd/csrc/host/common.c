@@ -2,6 +2,7 @@
 #include "common.h"
 
 #include <stdio.h>
+#include <stdlib.h>
 #include <string.h>
 
 int hype_nvar(int system)
@@ -181,4 +182,27 @@ double hype_cell_coord(const hype_problem *p, int axis, int i, int coord_mode)
     if (coord_mode == HYPE_COORD_DMDA && p->bc[2] != HYPE_BC_PERIODIC && p->bc[3] != HYPE_BC_PERIODIC)
         d = ((p->y_max + 0.5 * h) - (p->y_min + 0.5 * h)) / (double)(p->ny - 1);
     return p->y_min + 0.5 * h + (double)i * d;
+}
+
+/* XCD-aware order of a row-major tile list.  Workgroups of a launch are
+ * dispatched round-robin over the device's XCDs (workgroup i runs on XCD
+ * i mod nxcd), and every XCD has its own L2.  Giving each XCD a contiguous
+ * band of the row-major list (list[i] = band[i mod nxcd][i / nxcd]) keeps
+ * vertically adjacent tiles, which read each other's three halo rows, on
+ * the same L2. */
+int hype_xcd_order(hype_tile *tiles, int n, int nxcd)
+{
+    hype_tile *tmp;
+    int x, i, start = 0;
+    if (nxcd < 2 || n <= nxcd) return HYPE_OK;
+    tmp = (hype_tile *)malloc((size_t)n * sizeof(hype_tile));
+    if (!tmp) return HYPE_E_NOMEM;
+    memcpy(tmp, tiles, (size_t)n * sizeof(hype_tile));
+    for (x = 0; x < nxcd; ++x) {
+        const int cnt = n / nxcd + (x < n % nxcd ? 1 : 0); /* slots i = x, x + nxcd, ... below n */
+        for (i = 0; i < cnt; ++i) tiles[x + i * nxcd] = tmp[start + i];
+        start += cnt;
+    }
+    free(tmp);
+    return HYPE_OK;
 }

@@ -28,6 +28,11 @@ double hype_tile_cost(long tiles, int hmax, int simds, int occ);
  * flagged as leader, or -1 */
 int hype_make_tiles(int inst, int nx, int ny, int nsx, int nsy, hype_tile *tiles, int cap);
 
+/* devices dispatch workgroups round-robin over this many XCDs (MI355X) */
+#define HYPE_NXCD 8
+/* reorder a row-major tile list so each XCD gets a contiguous band */
+int hype_xcd_order(hype_tile *tiles, int n, int nxcd);
+
 /* face-key bits of the packed error word for an nx x ny grid */
 int hype_key_bits(int nx, int ny);
 

@@ -105,7 +105,7 @@ int hype_ctx_create(const hype_problem *p, hype_ctx **out)
     if (!tiles) { rc = HYPE_E_NOMEM; goto fail; }
     if (hype_make_tiles(0, p->nx, p->ny, hype_strips(p->nx), c->nsy, tiles, c->ntiles) != c->ntiles)
         rc = HYPE_E_INVALID;
-    else
+    else if ((rc = hype_xcd_order(tiles, c->ntiles, HYPE_NXCD)) == HYPE_OK)
         rc = hyped_memcpy_h2d(c->tiles_d, tiles, (size_t)c->ntiles * sizeof(hype_tile), c->stream);
     if (rc == HYPE_OK) rc = hyped_stream_sync(c->stream);
     free(tiles);
@@ -384,7 +384,8 @@ int hype_ctx_set_tiling(hype_ctx *c, int hrows, int grid)
         return HYPE_E_INVALID;
     }
     c->inst_h.nsy = nsy;
-    rc = hyped_memcpy_h2d(c->tiles_d, tiles, (size_t)c->ntiles * sizeof(hype_tile), c->stream);
+    rc = hype_xcd_order(tiles, c->ntiles, HYPE_NXCD);
+    if (rc == HYPE_OK) rc = hyped_memcpy_h2d(c->tiles_d, tiles, (size_t)c->ntiles * sizeof(hype_tile), c->stream);
     if (rc == HYPE_OK) rc = hyped_memcpy_h2d(c->inst_d, &c->inst_h, sizeof(hype_inst), c->stream);
     if (rc == HYPE_OK) rc = hyped_stream_sync(c->stream);
     free(tiles);
