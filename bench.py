@@ -122,19 +122,42 @@ def cpu_baseline(p, U0, nv, budget_s=12.0):
                       f"differencing a 1-step and a {k + 1}-step run)"}
 
 
-def load_traffic(workload_name):
-    """HBM bytes per stage-kernel launch measured by rocprofv3 (FETCH_SIZE and
-    WRITE_SIZE, each scaled by the same counter of a plain copy of known
-    size), read from the committed profile summary of this workload."""
+def profile_figure(workload_name, key):
+    """A figure measured by rocprofv3 for this workload's stage kernel, read
+    from the committed profile summary (a `key: value` line), or None."""
     path = os.path.join(ROOT, "profiles", f"stage_kernel_{workload_name}.md")
     try:
         with open(path) as f:
             for line in f:
-                if line.startswith("traffic_bytes_per_launch:"):
+                if line.startswith(key + ":"):
                     return int(line.split(":")[1])
     except OSError:
         pass
     return None
+
+
+def load_traffic(workload_name):
+    """HBM bytes per stage-kernel launch measured by rocprofv3 (FETCH_SIZE and
+    WRITE_SIZE, each scaled by the same counter of a plain copy of known
+    size)."""
+    return profile_figure(workload_name, "traffic_bytes_per_launch")
+
+
+CU_COUNT, CLOCK_GHZ = 256, 2.4  # MI355X
+
+
+def valu_roofline(workload_name, launch_s):
+    """FP64 VALU issue, the roof that binds the stage kernel: wave-instruction
+    issue slots per launch (SQ_ACTIVE_INST_VALU, transcendentals counted at
+    their quarter rate) over the launch time, against one wave instruction
+    per CU per cycle."""
+    slots = profile_figure(workload_name, "valu_issue_slots_per_launch")
+    if slots is None:
+        return None
+    achieved = slots / launch_s / 1e9
+    peak = CU_COUNT * CLOCK_GHZ
+    return {"bound": "fp64-valu-issue", "achieved": round(achieved, 2), "peak": peak,
+            "unit": "G wave-instructions/s", "frac": round(achieved / peak, 4), "issue_slots_per_launch": slots}
 
 
 def aggregate_roofline(cell_steps_by_variant, seconds):
@@ -453,6 +476,7 @@ def main():
                      "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": load_traffic(wname),
                      "kernel": "stage_kernel", "avg_launch_us": round(avg_launch_s * 1e6, 3),
                      "algorithmic_bytes_per_launch": int(bytes_launch)},
+        "valu_roofline": valu_roofline(wname, avg_launch_s),
         "final_time": st.t,
         "pcie_inclusive": {"rhs_call_ms": round(rhs_s * 1e3, 4), "nstage": nstage,
                            "equivalent_value": round(cells / (rhs_s * nstage) / 1e6, 3),
