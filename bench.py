@@ -383,6 +383,52 @@ def calibrate_copy(mib):
                       "GBps": 2 * n * 8 / ms / 1e6}))
 
 
+MOMENTA = {H.EULER: [(1, 2)], H.KAPILA: [(2, 3)], H.BAER_NUNZIATO: [(1, 2), (5, 6)]}
+
+
+def rel_dev(a, b, system):
+    """max over components of ||a_c - b_c||_inf / ||b_c||_inf, each momentum
+    pair scaled by the maximum of its vector magnitude (a transverse momentum
+    that is physically still zero holds only rounding noise)"""
+    scale = [max(float(np.max(np.abs(b[..., c]))), 1e-300) for c in range(b.shape[-1])]
+    for (cu, cv) in MOMENTA[system]:
+        scale[cu] = scale[cv] = max(float(np.max(np.hypot(b[..., cu], b[..., cv]))), 1e-300)
+    return max(float(np.max(np.abs(a[..., c] - b[..., c]))) / scale[c] for c in range(b.shape[-1]))
+
+
+def timed_run(p, U0, steps, args, pcie=True):
+    """Warm up, then time `steps` device steps from U0 with HIP events on the
+    context's stream.  Returns (ms, status, final state, tiling, seconds per
+    host-buffer RHS call)."""
+    flags = H.TS_CLIP | (H.TS_GRAPH if args.graph else 0)
+    rhs_s = None
+    with H.Context(p) as ctx:
+        if args.hrows:
+            ctx.set_tiling(args.hrows, 0)
+        # warmup (untimed) from the initial state, then restart from it
+        ctx.set_state(U0)
+        ctx.begin(0.0, float("inf"), max(args.warmup, 1), flags)
+        ctx.advance(max(args.warmup, 1))
+        ctx.sync()
+        ctx.set_state(U0)
+        ctx.begin(0.0, float("inf"), steps, flags)
+        ctx.sync()
+        ms = ctx.advance_timed(steps)
+        st = ctx.sync()
+        U_end = ctx.get_state()
+        tiling = ctx.tiling()
+        if pcie:
+            # the drop-in callback with host buffers (TSRHSFunction on host
+            # Vecs): every call uploads U and downloads the RHS over PCIe
+            ctx.rhs(U0)
+            nrhs = 10
+            t0 = time.perf_counter()
+            for _ in range(nrhs):
+                ctx.rhs(U0)
+            rhs_s = (time.perf_counter() - t0) / nrhs
+    return ms, st, U_end, tiling, rhs_s
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -401,6 +447,8 @@ def main():
     ap.add_argument("--active-cells", type=int, default=0,
                     help="served: cells advanced per round (0 = enough to fill the device, -1 = no cap)")
     ap.add_argument("--seed", type=int, default=2024)
+    ap.add_argument("--no-fma", action="store_true",
+                    help="single-grid configs: skip timing the FMA-contracted build (HYPE_OPT_FMA) beside the exact one")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="run only a plain device copy of MIB MiB (counter calibration)")
     args = ap.parse_args()
@@ -423,34 +471,23 @@ def main():
     steps = args.steps if args.steps is not None else steps_default
     nv = H.NVAR[p.system]
     cells = p.nx * p.ny
-    flags = H.TS_CLIP | (H.TS_GRAPH if args.graph else 0)
-    with H.Context(p) as ctx:
-        if args.hrows:
-            ctx.set_tiling(args.hrows, 0)
-        # warmup (untimed) from the initial state, then restart from it
-        ctx.set_state(U0)
-        ctx.begin(0.0, float("inf"), max(args.warmup, 1), flags)
-        ctx.advance(max(args.warmup, 1))
-        ctx.sync()
-        ctx.set_state(U0)
-        ctx.begin(0.0, float("inf"), steps, flags)
-        ctx.sync()
-        ms = ctx.advance_timed(steps)
-        st = ctx.sync()
-        hrows, ntiles, grid = ctx.tiling()
-        # the drop-in callback with host buffers (TSRHSFunction on host Vecs):
-        # every call uploads U and downloads the RHS over PCIe
-        ctx.rhs(U0)
-        nrhs = 10
-        t0 = time.perf_counter()
-        for _ in range(nrhs):
-            ctx.rhs(U0)
-        rhs_s = (time.perf_counter() - t0) / nrhs
+    ms, st, U_end, (hrows, ntiles, grid), rhs_s = timed_run(p, U0, steps, args)
+    fma_variant = None
+    if not args.no_fma:
+        # the opt-in FMA-contracted build (HYPE_OPT_FMA) on the same run,
+        # reported beside the exact build, which is the headline
+        p_fma = H.Problem.from_buffer_copy(p)
+        p_fma.options = H.OPT_FMA
+        ms_f, st_f, U_f, _, _ = timed_run(p_fma, U0, steps, args, pcie=False)
+        fma_variant = {"value": round(cells * steps / (ms_f / 1e3) / 1e6, 3),
+                       "max_rel_dev_vs_exact": rel_dev(U_f, U_end, p.system),
+                       "final_time_rel_dev": abs(st_f.t - st.t) / st.t,
+                       "note": "HYPE_OPT_FMA: same kernels built with FMA contraction; not bitwise"}
     if st.steps != steps:
         raise SystemExit(f"only {st.steps} of {steps} steps ran")
     sec = ms / 1e3
     mcups = cells * steps / sec / 1e6
-    nstage = 4 if p.order == 2 else (2 if p.order == 1 else 1)
+    nstage = {0: 1, 1: 2, 2: 4, 3: 10}[p.order]
     launches = steps * nstage
     avg_launch_s = sec / launches
     bytes_launch = cells * bytes_per_cell_step(nv, p.order) / nstage
@@ -471,9 +508,11 @@ def main():
         "config": {"workload": wname, "cells": cells, "nvar": nv, "order": p.order,
                    "path": "RHSFunctionPrim" if p.recons_prim else "RHSFunction",
                    "scheme": "WENO3+Rusanov+SSPRK(4,3)" if p.order == 2 else f"order{p.order}",
-                   "cfl": p.cfl, "tile_rows": hrows, "tiles": ntiles, "graph": bool(args.graph)},
+                   "cfl": p.cfl, "tile_rows": hrows, "tiles": ntiles, "graph": bool(args.graph),
+                   "kernel_build": "exact"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": load_traffic(wname),
+                     "frac": round(achieved / HBM_PEAK_GBS, 5),
+                     "traffic": load_traffic(wname),
                      "kernel": "stage_kernel", "avg_launch_us": round(avg_launch_s * 1e6, 3),
                      "algorithmic_bytes_per_launch": int(bytes_launch)},
         "valu_roofline": valu_roofline(wname, avg_launch_s),
@@ -482,6 +521,8 @@ def main():
                            "equivalent_value": round(cells / (rhs_s * nstage) / 1e6, 3),
                            "note": "hype_rhs on host buffers (upload U, download RHS), per RK stage"},
     }
+    if fma_variant is not None:
+        out["fma_variant"] = fma_variant
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(p, U0, nv)
     print(json.dumps(out))

@@ -15,6 +15,10 @@ hipError_t launch_stage(const hype_launch& L, hipStream_t s);
 int blocks_per_cu(const hype_variant& v);
 hipError_t launch_cons2prim(int system, const double* U, double* W, long n, Eos e, hipStream_t s);
 }  // namespace hype
+namespace hype_fc {  // the same kernels compiled with FMA contraction (sweep_fc.hip)
+hipError_t launch_stage(const hype_launch& L, hipStream_t s);
+int blocks_per_cu(const hype_variant& v);
+}  // namespace hype_fc
 
 static thread_local char g_err[256];
 
@@ -123,7 +127,8 @@ int hyped_stream_sync(void* s) {
 }
 
 int hyped_launch(const hype_launch* L, void* s) {
-  HCHECK(hype::launch_stage(*L, (hipStream_t)s), "stage kernel launch");
+  HCHECK(L->v.fma ? hype_fc::launch_stage(*L, (hipStream_t)s) : hype::launch_stage(*L, (hipStream_t)s),
+         "stage kernel launch");
   return HYPE_OK;
 }
 
@@ -146,7 +151,7 @@ int hyped_num_cus(void) {
   return n;
 }
 
-int hyped_blocks_per_cu(const hype_variant* v) { return hype::blocks_per_cu(*v); }
+int hyped_blocks_per_cu(const hype_variant* v) { return v->fma ? hype_fc::blocks_per_cu(*v) : hype::blocks_per_cu(*v); }
 
 int hyped_graph_begin(void* s) {
   HCHECK(hipStreamBeginCapture((hipStream_t)s, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");

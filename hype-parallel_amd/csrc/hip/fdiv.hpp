@@ -29,9 +29,12 @@
 // Callers check the range condition (usually once for a group of
 // divisions) and fall back to plain `/` when it fails.
 #pragma once
+#ifndef HYPE_KNS
+#define HYPE_KNS hype  // kernel namespace; the contracted build uses its own
+#endif
 #include <hip/hip_runtime.h>
 
-namespace hype {
+namespace HYPE_KNS {
 
 constexpr double FDIV_HI = 0x1p380;
 constexpr double FDIV_LO = 0x1p-380;
@@ -64,4 +67,4 @@ __device__ __forceinline__ bool fdiv_in_range(double x) {
   return a >= FDIV_LO && a <= FDIV_HI;
 }
 
-}  // namespace hype
+}  // namespace HYPE_KNS

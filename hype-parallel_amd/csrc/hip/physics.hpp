@@ -8,9 +8,12 @@
 // (1,0); DIR 1: y-face, normal (0,1)); products with the zero normal
 // component are kept so that even the sign of a zero result matches.
 #pragma once
+#ifndef HYPE_KNS
+#define HYPE_KNS hype  // kernel namespace; the contracted build uses its own
+#endif
 #include <hip/hip_runtime.h>
 
-namespace hype {
+namespace HYPE_KNS {
 
 // hype.h:50-52
 constexpr double PRS_FLOOR = 1.0e-12;
@@ -653,4 +656,4 @@ __device__ __forceinline__ double riemann(const Eos& e, const double* L, const d
   return s;
 }
 
-}  // namespace hype
+}  // namespace HYPE_KNS

@@ -9,11 +9,14 @@
 // coefficients (the running sum starts at +0.0 and can never become -0.0, so
 // adding a signed zero is the identity).
 #pragma once
+#ifndef HYPE_KNS
+#define HYPE_KNS hype  // kernel namespace; the contracted build uses its own
+#endif
 #include <hip/hip_runtime.h>
 
 #include "fdiv.hpp"
 
-namespace hype {
+namespace HYPE_KNS {
 
 constexpr double GPT = 0.28867513459481287;  // hype.c:214-230
 
@@ -362,4 +365,4 @@ __device__ __forceinline__ void reconstruct(const double* U_x, const double* U_y
   }
 }
 
-}  // namespace hype
+}  // namespace HYPE_KNS

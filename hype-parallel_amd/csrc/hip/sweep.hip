@@ -31,7 +31,7 @@
 #include "physics.hpp"
 #include "recon.hpp"
 
-namespace hype {
+namespace HYPE_KNS {
 
 // --------------------------------------------------------------- schemes --
 // Stage table of the SSP scheme PETSc uses for order N (hype.c:362-382).
@@ -650,13 +650,14 @@ hipError_t launch_cons2prim(int system, const double* U, double* W, long n, Eos 
   return hipGetLastError();
 }
 
-}  // namespace hype
+}  // namespace HYPE_KNS
 
+#ifndef HYPE_CONTRACTED_BUILD
 // The stage table as the host sees it (tests replay the buffer algebra on
 // the CPU against the oracle's TSSSP steps).
 extern "C" int hyped_stage_table(int order, int s, int* ibuf, double* coef) {
-  if (order < 0 || order > 3 || s < 0 || s >= hype::num_stages_dev(order)) return -1;
-  const hype::StageDesc d = hype::stage_desc(order, s);
+  if (order < 0 || order > 3 || s < 0 || s >= HYPE_KNS::num_stages_dev(order)) return -1;
+  const HYPE_KNS::StageDesc d = HYPE_KNS::stage_desc(order, s);
   ibuf[0] = d.in;
   ibuf[1] = d.out;
   ibuf[2] = d.xb;
@@ -665,5 +666,6 @@ extern "C" int hyped_stage_table(int order, int s, int* ibuf, double* coef) {
   coef[0] = d.a;
   coef[1] = d.g;
   coef[2] = d.bden;
-  return hype::num_stages_dev(order);
+  return HYPE_KNS::num_stages_dev(order);
 }
+#endif

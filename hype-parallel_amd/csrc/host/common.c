@@ -28,6 +28,7 @@ int hype_check_problem(const hype_problem *p, int *nvar)
         return HYPE_E_UNSUPPORTED;
     if (!(p->x_max > p->x_min) || !(p->y_max > p->y_min)) return HYPE_E_INVALID;
     if (!(p->cfl > 0.0)) return HYPE_E_INVALID;
+    if (p->options & ~HYPE_OPT_FMA) return HYPE_E_INVALID;
     for (s = 0; s < 4; ++s) {
         if (p->bc[s] == HYPE_BC_WALL || p->bc[s] == HYPE_BC_INLET) return HYPE_E_UNSUPPORTED;
         if (p->bc[s] != HYPE_BC_PERIODIC && p->bc[s] != HYPE_BC_REFLECTIVE &&

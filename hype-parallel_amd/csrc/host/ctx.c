@@ -74,6 +74,7 @@ int hype_ctx_create(const hype_problem *p, hype_ctx **out)
         v.system = p->system;
         v.prim = p->recons_prim;
         v.order = p->order;
+        v.fma = (p->options & HYPE_OPT_FMA) != 0;
         c->slots = hyped_num_cus() * hyped_blocks_per_cu(&v);
         if (c->slots < 1) c->slots = 1024;
     }
@@ -128,6 +129,7 @@ static hype_launch base_launch(hype_ctx *c, int mode, int stage)
     L.v.system = c->P.system;
     L.v.prim = c->P.recons_prim;
     L.v.order = c->P.order;
+    L.v.fma = (c->P.options & HYPE_OPT_FMA) != 0;
     L.insts = c->inst_d;
     L.tiles = c->tiles_d;
     L.ntiles = c->ntiles;

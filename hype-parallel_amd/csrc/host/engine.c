@@ -62,7 +62,7 @@ void eng_free(hype_engine *E)
 
 static int same_variant(const hype_variant *a, const hype_variant *b)
 {
-    return a->system == b->system && a->prim == b->prim && a->order == b->order;
+    return a->system == b->system && a->prim == b->prim && a->order == b->order && a->fma == b->fma;
 }
 
 static int find_group(hype_engine *E, const hype_variant *v)
@@ -108,6 +108,7 @@ int eng_attach(hype_engine *E, int slot, const hype_problem *p, double *U, doubl
     v.system = p->system;
     v.prim = p->recons_prim;
     v.order = p->order;
+    v.fma = (p->options & HYPE_OPT_FMA) != 0;
     g = find_group(E, &v);
     if (g < 0) return HYPE_E_DEVICE;
     E->P[slot] = *p;

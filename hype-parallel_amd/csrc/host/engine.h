@@ -15,7 +15,7 @@
 
 #include "common.h"
 
-#define HYPE_MAX_GROUPS 18
+#define HYPE_MAX_GROUPS 36  /* system x path x order 1..3 x contraction */
 
 typedef struct {
     hype_variant v;

@@ -82,6 +82,7 @@ typedef struct hype_tile {
 /* kernel variant: system x path x order */
 typedef struct hype_variant {
     int system, prim, order;
+    int fma;               /* 1: the FMA-contracted kernel build (HYPE_OPT_FMA) */
 } hype_variant;
 
 typedef struct hype_launch {

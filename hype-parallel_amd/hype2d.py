@@ -21,6 +21,7 @@ OK, E_INVALID, E_UNSUPPORTED, E_DEVICE, E_NOMEM, E_INADMISSIBLE, E_BUSY, E_NOTFO
 EULER, KAPILA, BAER_NUNZIATO = 0, 1, 2
 BC_WALL, BC_INLET, BC_PERIODIC, BC_REFLECTIVE, BC_TRANSMISSIVE = range(5)
 QUIRK_BN_P2_IS_PHI = 0x1
+OPT_FMA = 0x1
 MEM_HOST, MEM_DEVICE = 0, 1
 TS_CLIP, TS_GRAPH = 0x1, 0x2
 NVAR = {EULER: 4, KAPILA: 6, BAER_NUNZIATO: 9}
@@ -51,6 +52,7 @@ class Problem(C.Structure):
         ("pi1", C.c_double),
         ("pi2", C.c_double),
         ("quirks", C.c_uint),
+        ("options", C.c_uint),
     ]
 
 
@@ -104,7 +106,7 @@ class RequestInfo(C.Structure):
 
 
 def make_problem(system, nx, ny, *, order=2, prim=True, domain=(0.0, 1.0, 0.0, 1.0), cfl=0.5,
-                 bc=(BC_TRANSMISSIVE,) * 4, gamma1=1.4, gamma2=1.4, pi1=0.0, pi2=0.0, quirks=0):
+                 bc=(BC_TRANSMISSIVE,) * 4, gamma1=1.4, gamma2=1.4, pi1=0.0, pi2=0.0, quirks=0, fma=False):
     p = Problem()
     p.system, p.order, p.recons_prim = system, order, 1 if prim else 0
     p.nx, p.ny = nx, ny
@@ -114,6 +116,7 @@ def make_problem(system, nx, ny, *, order=2, prim=True, domain=(0.0, 1.0, 0.0, 1
         p.bc[k] = bc[k]
     p.gamma1, p.gamma2, p.pi1, p.pi2 = gamma1, gamma2, pi1, pi2
     p.quirks = quirks
+    p.options = OPT_FMA if fma else 0
     return p
 
 

@@ -100,3 +100,12 @@ def test_smooth_vortex_profile():
     r = np.hypot(X, Y)
     a = 0.25 + 0.25 * np.vectorize(erfc)((r - 1.0) / 0.3)
     assert np.max(np.abs(U[..., 5] - a)) < 0.01  # cell average vs centre value, O(h^2)
+
+
+def test_problem_options_accept_fma_and_reject_unknown_bits():
+    p = H.make_problem(H.EULER, 8, 6, domain=(0, 1, 0, 0.75), gamma1=1.4, fma=True)
+    assert p.options == H.OPT_FMA
+    H.initial_state(p, H.IC_UNIFORM, [1.0, 0.0, 0.0, 1.0])  # validated, accepted
+    p.options = 0x2
+    with pytest.raises(H.HypeError):
+        H.initial_state(p, H.IC_UNIFORM, [1.0, 0.0, 0.0, 1.0])
