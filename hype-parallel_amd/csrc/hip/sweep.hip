@@ -219,7 +219,10 @@ struct Tile {
     auto W_ = [&](int s, int c, int k) -> double& { return win[(s * NV + c) * COLS + k]; };
 
     // ---- row loader ----------------------------------------------------
-    auto fetch = [&](int rr, int k, double* q) {
+    // fetch issues the loads of one cell (source cell from the boundary
+    // index maps) and returns its reflection flags; put applies them, so a
+    // row's loads need not complete before the next row's are issued
+    auto fetch = [&](int rr, int k, double* q) -> int {
       const int cc = min(x0 - 3 + k, nx + 2);
       bool fx = false, fy = false;
       const int sc = bc_map(cc, nx, bcL, bcR, fx);
@@ -227,16 +230,17 @@ struct Tile {
       const double* p = Uin + ((size_t)sr * nx + sc) * NV;
 #pragma unroll
       for (int c = 0; c < NV; ++c) q[c] = p[c];
-      if (fx) {
+      return (fx ? 1 : 0) | (fy ? 2 : 0);
+    };
+    auto put = [&](int s, int k, double* q, int refl) {
+      if (refl & 1) {
         q[S::REFX[0]] = -q[S::REFX[0]];
         if (S::REFX[1] >= 0) q[S::REFX[1]] = -q[S::REFX[1]];
       }
-      if (fy) {
+      if (refl & 2) {
         q[S::REFY[0]] = -q[S::REFY[0]];
         if (S::REFY[1] >= 0) q[S::REFY[1]] = -q[S::REFY[1]];
       }
-    };
-    auto put = [&](int s, int k, const double* q) {
       double v[NV];
       if (PRIM) {
         S::cons2prim(eos, q, v);
@@ -248,13 +252,21 @@ struct Tile {
       for (int c = 0; c < NV; ++c) W_(s, c, k) = v[c];
     };
 
-    // prologue: rows y0-3 .. y0+1
-    for (int rr = y0 - 3; rr <= y0 + 1; ++rr) {
-      double qa[NV], qb[NV];
-      fetch(rr, lane, qa);
-      if (lane < 4) fetch(rr, lane + HYPE_WAVE, qb);
-      put(slot(rr), lane, qa);
-      if (lane < 4) put(slot(rr), lane + HYPE_WAVE, qb);
+    // prologue: rows y0-3 .. y0+1, all loads in flight before the first store
+    {
+      double qa[5][NV], qb[5][NV];
+      int fa[5], fb[5];
+#pragma unroll
+      for (int i = 0; i < 5; ++i) {
+        fa[i] = fetch(y0 - 3 + i, lane, qa[i]);
+        fb[i] = 0;
+        if (lane < 4) fb[i] = fetch(y0 - 3 + i, lane + HYPE_WAVE, qb[i]);
+      }
+#pragma unroll
+      for (int i = 0; i < 5; ++i) {
+        put(slot(y0 - 3 + i), lane, qa[i], fa[i]);
+        if (lane < 4) put(slot(y0 - 3 + i), lane + HYPE_WAVE, qb[i], fb[i]);
+      }
     }
     __syncthreads();
 
@@ -282,9 +294,10 @@ struct Tile {
       // prefetch the row that enters the window after this iteration
       const bool more = (r + 3 <= y0 + H + 2);
       double qa[NV], qb[NV];
+      int fa = 0, fb = 0;
       if (more) {
-        fetch(r + 3, lane, qa);
-        if (lane < 4) fetch(r + 3, lane + HYPE_WAVE, qb);
+        fa = fetch(r + 3, lane, qa);
+        if (lane < 4) fb = fetch(r + 3, lane + HYPE_WAVE, qb);
       }
 
       const bool ownedRow = (r >= y0) && (r < y0 + H);
@@ -360,8 +373,8 @@ struct Tile {
       // slot (the wave's own reads above precede this store in program order)
       if (more) {
         const int sn = slot(r + 3);
-        put(sn, lane, qa);
-        if (lane < 4) put(sn, lane + HYPE_WAVE, qb);
+        put(sn, lane, qa, fa);
+        if (lane < 4) put(sn, lane + HYPE_WAVE, qb, fb);
       }
 
       // ---- volume non-conservative product (rhs_function_prim.c:461-479)
