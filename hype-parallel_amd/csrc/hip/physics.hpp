@@ -50,6 +50,10 @@ struct Nrm {
 struct Euler {
   static constexpr int NV = 4;
   static constexpr int WAVES_PER_EU = 2;  // register budget of the stage kernel: 256 per lane
+  // one wave per workgroup: on the 256^2 grids this system is benchmarked on,
+  // split tiles give fewer, two-wave workgroups, and the dispatcher then puts
+  // two waves on one SIMD while others idle (measured 17 % slower)
+  static constexpr bool PAIR = false;
   // no sound speed is shared between the admissibility check and the flux
   static constexpr bool HAS_ACACHE = false;
   static constexpr int REFX[2] = {1, -1};  // reflective x-wall negates comp 1
@@ -149,6 +153,9 @@ struct Euler {
 struct Kapila {
   static constexpr int NV = 6;
   static constexpr int WAVES_PER_EU = 1;
+  // two waves per workgroup share each tile: one reconstructed ring row per
+  // wave saved (1024 x 512 shock-bubble: 952 -> 987 Mcell-updates/s)
+  static constexpr bool PAIR = true;
   // the admissibility check at a node computes the a_sq the primitive flux
   // at that point needs (kapila_5eqn.c:347-349 and 468-470 are the same
   // expression): the stage kernel hands it over instead of recomputing
@@ -419,6 +426,9 @@ struct Kapila {
 struct BN {
   static constexpr int NV = 9;
   static constexpr int WAVES_PER_EU = 1;
+  // one wave per workgroup: two windows of nine components would halve the
+  // workgroups an LDS of 160 KB holds
+  static constexpr bool PAIR = false;
   // no sound speed is shared between the admissibility check and the flux
   static constexpr bool HAS_ACACHE = false;
   static constexpr int REFX[2] = {1, 5};

@@ -21,6 +21,15 @@
 //     fused; the stage output goes straight to HBM, and the wave's largest
 //     wave speed feeds the device-wide CFL maximum (:617-623).
 //
+// Systems with S::PAIR run two waves per workgroup.  A tile of two or more
+// rows is split at its middle row ym; both waves march up.  The lower wave
+// starts on the ring row below the tile as above and stops at row ym-1.
+// The upper wave starts at ym with no ring row below it.  It sets aside row
+// ym's bottom-face values and the assembly of row ym until both halves are
+// done; then the two waves swap the values either side of face ym-1/2
+// through LDS (one workgroup barrier), each computes that face's flux and
+// assembles its boundary row.  One reconstructed ring row per wave is saved.
+//
 // Arithmetic follows the reference's expression trees and is compiled
 // without FMA contraction, so results equal the CPU restatement bit for bit.
 #include <hip/hip_runtime.h>
@@ -118,6 +127,14 @@ __device__ __forceinline__ unsigned long long wave_min_u64(unsigned long long v)
   return v;
 }
 
+// orders a wave's own LDS accesses across lanes: LDS operations of one wave
+// complete in issue order, so only the compiler's code motion is fenced
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 constexpr int NSLOT = 5;
 constexpr int COLS = HYPE_WAVE + 4;
 
@@ -126,6 +143,18 @@ struct Tile {
   static constexpr int NV = S::NV;
   using Qd = Quad<N>;
   static constexpr int NDOF = Qd::NDOF, NGF = Qd::NGF, NGV = Qd::NGV, NNODE = Qd::NNODE;
+  static constexpr bool PAIR = S::PAIR;
+  static constexpr int WAVES = PAIR ? 2 : 1;
+  // the admissibility check's a_sq at each node goes on to the flux there
+  static constexpr bool ACACHE = PRIM && S::HAS_ACACHE && N == 2;
+  // LDS per workgroup, in doubles: one window per wave, then (paired) the
+  // two exchange areas (NGF x (NV + 1) values per lane: face values and the
+  // a_sq handed over with them) and the upper wave's deferred row state
+  // (G, E, x part and volume term of row ym: 4 x NV per lane)
+  static constexpr int WIN = NSLOT * NV * COLS;
+  static constexpr int XCH = NGF * (NV + 1) * HYPE_WAVE;
+  static constexpr int DEFER = 4 * NV * HYPE_WAVE;
+  static constexpr int LDS = WAVES * WIN + (PAIR ? 2 * XCH + DEFER : 0);
 
   // value of component c at node/face point p from its modal coefficients
   // (rhs_function_prim.c:382-388, 425-440)
@@ -157,7 +186,7 @@ struct Tile {
     }
   }
 
-  __device__ static void run(const hype_launch& L, int tile_id, double* win, int lane) {
+  __device__ static void run(const hype_launch& L, int tile_id, double* lds, int lane, int wave) {
     const hype_tile T = L.tiles[tile_id];
     const hype_inst* I = &L.insts[T.inst];
     hype_dstate* st = I->st;
@@ -165,7 +194,8 @@ struct Tile {
     const int bcL = I->bc[0], bcR = I->bc[1], bcB = I->bc[2], bcT = I->bc[3];
     const double r1_h = I->r1_h;
     const Eos eos{I->g1, I->g2, I->p1, I->p2, (I->quirks & HYPE_QUIRK_BN_P2_IS_PHI) != 0};
-    const bool leader = (T.flags & 1) != 0;
+    // the first tile of an instance writes its time state (lower wave only)
+    const bool leader = (T.flags & 1) != 0 && wave == 0;
     const int mode = L.mode;
 
     // ---- time state (single writer per field per launch) -------------
@@ -213,10 +243,34 @@ struct Tile {
     const int col = x0 - 1 + lane;
     const bool owned = lane >= 1 && lane <= W;
     const bool xlane = lane >= 1 && lane <= W + 1;
-    const int cw = lane + 2;                 // window column of this lane's cell
-    const int rbase = y0 - 3;                // window row of slot 0
+    const int cw = lane + 2;  // window column of this lane's cell
+    // split tile (paired systems, two or more rows): the lower wave owns
+    // rows [y0, ym), the upper wave [ym, y0 + H); a one-row tile is the
+    // lower wave's alone
+    const bool split = PAIR && H >= 2;
+    if (PAIR && !split && wave != 0) return;
+    const int ym = y0 + (H + 1) / 2;
+    const bool upper = split && wave == 1;
+    const int own_lo = upper ? ym : y0;
+    const int own_hi = (split && !upper) ? ym : y0 + H;
+    const int r_first = upper ? ym : y0 - 1;                // lower ring row unless upper half
+    const int r_last = (split && !upper) ? ym - 1 : y0 + H;  // upper ring row unless lower half
+    const int y_first = upper ? ym + 1 : y0;                // first row whose face r-1/2 is computed
+    const int a_first = upper ? ym + 2 : y0 + 1;            // first iteration that assembles row r-1
+    double* win = lds + (PAIR ? wave * WIN : 0);
+    double* xch = lds + WAVES * WIN;  // [0]: row ym-1 top values, [1]: row ym bottom values
+    double* defer = xch + 2 * XCH;
+    auto X_ = [&](int side, int q, int c) -> double& { return xch[side * XCH + (q * (NV + 1) + c) * HYPE_WAVE + lane]; };
+    auto D_ = [&](int k, int c) -> double& { return defer[(k * NV + c) * HYPE_WAVE + lane]; };
+    const int rbase = r_first - 2;  // window row of slot 0
     auto slot = [&](int r) { return (r - rbase) % NSLOT; };
     auto W_ = [&](int s, int c, int k) -> double& { return win[(s * NV + c) * COLS + k]; };
+    auto sync_window = [&]() {
+      if constexpr (PAIR)
+        wave_sync();
+      else
+        __syncthreads();
+    };
 
     // ---- row loader ----------------------------------------------------
     // fetch issues the loads of one cell (source cell from the boundary
@@ -252,23 +306,24 @@ struct Tile {
       for (int c = 0; c < NV; ++c) W_(s, c, k) = v[c];
     };
 
-    // prologue: rows y0-3 .. y0+1, all loads in flight before the first store
+    // prologue: rows r_first-2 .. r_first+2, all loads in flight before the
+    // first store
     {
       double qa[5][NV], qb[5][NV];
       int fa[5], fb[5];
 #pragma unroll
       for (int i = 0; i < 5; ++i) {
-        fa[i] = fetch(y0 - 3 + i, lane, qa[i]);
+        fa[i] = fetch(rbase + i, lane, qa[i]);
         fb[i] = 0;
-        if (lane < 4) fb[i] = fetch(y0 - 3 + i, lane + HYPE_WAVE, qb[i]);
+        if (lane < 4) fb[i] = fetch(rbase + i, lane + HYPE_WAVE, qb[i]);
       }
 #pragma unroll
       for (int i = 0; i < 5; ++i) {
-        put(slot(y0 - 3 + i), lane, qa[i], fa[i]);
-        if (lane < 4) put(slot(y0 - 3 + i), lane + HYPE_WAVE, qb[i], fb[i]);
+        put(slot(rbase + i), lane, qa[i], fa[i]);
+        if (lane < 4) put(slot(rbase + i), lane + HYPE_WAVE, qb[i], fb[i]);
       }
     }
-    __syncthreads();
+    sync_window();
 
     // ---- carried state -------------------------------------------------
     double topPrev[NGF][NV];
@@ -290,9 +345,83 @@ struct Tile {
     const int seq = (mode == HYPE_MODE_STAGE) ? L.stage + 1 : 0;
     const double WF = Qd::WF, WV = Qd::WV;
 
-    for (int r = y0 - 1; r <= y0 + H; ++r) {
+    // flux + jump at the y-face below row fr from the values either side
+    // (lower side: top values of row fr-1; upper side: bottom values of fr)
+    auto yface = [&](const double (&Lv)[NGF][NV], const double (&Rv)[NGF][NV], const double* aL, const double* aR,
+                     int fr, double* G, double* E) {
+#pragma unroll
+      for (int c = 0; c < NV; ++c) {
+        G[c] = 0.0;
+        E[c] = 0.0;
+      }
+#pragma unroll
+      for (int q = 0; q < NGF; ++q) {
+        double Fq[NV], Dq[NV];
+        Bad b{0, 0.0};
+        const double sp = riemann<S, PRIM, 1>(eos, Lv[q], Rv[q], Fq, Dq, b, aL[q], aR[q]);
+        if (owned) {
+          if (sp > smax) smax = sp;
+          if (b.kind) {
+            const long key = (long)ny * (nx + 1) * NGF + ((long)fr * nx + col) * NGF + q;
+            const unsigned long long pk = pack_err(seq, key, b.kind, b.val, I->keybits);
+            if (pk < err) err = pk;
+          }
+        }
+#pragma unroll
+        for (int c = 0; c < NV; ++c) {
+          G[c] = G[c] + WF * Fq[c];
+          if (S::DNZ[c]) E[c] = E[c] + WF * Dq[c];
+        }
+      }
+    };
+
+    // assembly + stage update of cell (col, row) from the terms of its top
+    // and bottom faces, its x-face part and its volume term
+    // (rhs_function_prim.c:589-602)
+    auto assemble = [&](int row, const double* Gt, const double* Gb, const double* Et, const double* Eb,
+                        const double* xp, const double* nc) {
+      if (!owned || mode == HYPE_MODE_PRIME) return;
+      const size_t idx = ((size_t)row * nx + col) * NV;
+      double rhs[NV];
+#pragma unroll
+      for (int c = 0; c < NV; ++c) {
+        const double Y = Gt[c] - Gb[c] + 0.5 * (Et[c] + Eb[c]);
+        const double t = xp[c] - r1_h * Y;
+        rhs[c] = ((S::HAS_NCP && S::DNZ[c]) ? nc[c] : 0.0) + t;
+      }
+      if (mode == HYPE_MODE_RHS) {
+        double* out = L.rhs_out ? L.rhs_out : I->buf[1];
+#pragma unroll
+        for (int c = 0; c < NV; ++c) out[idx + c] = rhs[c];
+      } else {
+        double* out = I->buf[sd.out];
+        if (sd.op == 0) {
+#pragma unroll
+          for (int c = 0; c < NV; ++c) out[idx + c] = Uin[idx + c] + cdt * rhs[c];
+        } else if (sd.op == 2) {
+          // RK104 after its fifth stage: w0 += dt/6 F (VecAXPY); w1 =
+          // 1/25 U + 9/25 w0 (VecAXPBYPCZ, c = 0) into U; w0 = 15 w1 - 5 w0
+          // (VecAXPBY) into A
+          double* A = I->buf[1];
+#pragma unroll
+          for (int c = 0; c < NV; ++c) {
+            const double w0 = Uin[idx + c] + cdt * rhs[c];
+            const double w1 = sd.a * out[idx + c] + sd.g * w0;
+            out[idx + c] = w1;
+            A[idx + c] = 15.0 * w1 + -5.0 * w0;
+          }
+        } else {
+          const double* X = I->buf[sd.xb];
+          const double* Z = I->buf[sd.zb];
+#pragma unroll
+          for (int c = 0; c < NV; ++c) out[idx + c] = sd.a * X[idx + c] + cdt * rhs[c] + sd.g * Z[idx + c];
+        }
+      }
+    };
+
+    for (int r = r_first; r <= r_last; ++r) {
       // prefetch the row that enters the window after this iteration
-      const bool more = (r + 3 <= y0 + H + 2);
+      const bool more = (r + 3 <= r_last + 2);
       double qa[NV], qb[NV];
       int fa = 0, fb = 0;
       if (more) {
@@ -300,9 +429,9 @@ struct Tile {
         if (lane < 4) fb = fetch(r + 3, lane + HYPE_WAVE, qb);
       }
 
-      const bool ownedRow = (r >= y0) && (r < y0 + H);
-      const bool doY = (r >= y0);
-      const bool doAsm = (r >= y0 + 1);
+      const bool ownedRow = (r >= own_lo) && (r < own_hi);
+      const bool doY = (r >= y_first);
+      const bool doAsm = (r >= a_first);
       const int sm2 = slot(r - 2), sm1 = slot(r - 1), s0 = slot(r), sp1 = slot(r + 1), sp2 = slot(r + 2);
 
       // ---- reconstruction: modal coefficients of every component --------
@@ -336,7 +465,6 @@ struct Tile {
         for (int c = 0; c < NV; ++c) fv[p][c] = at_node(co, c, p);
       // admissibility at the nodes (rhs_function_prim.c:393-404); with
       // ACACHE the check's a_sq at each node goes on to the flux there
-      constexpr bool ACACHE = PRIM && S::HAS_ACACHE && N == 2;
       double an[ACACHE ? NNODE : 1];
       bool pad = true;
 #pragma unroll
@@ -446,77 +574,45 @@ struct Tile {
 
       // ---- y-face r-1/2 (rhs_function_prim.c:537-585) --------------------
       double Gy[NV], Ey[NV];
-#pragma unroll
-      for (int c = 0; c < NV; ++c) {
-        Gy[c] = 0.0;
-        Ey[c] = 0.0;
-      }
       if (doY) {
+        double an_b[NGF];
 #pragma unroll
-        for (int q = 0; q < NGF; ++q) {
-          double Fq[NV], Dq[NV];
-          Bad b{0, 0.0};
-          double ab = -1.0, at = -1.0;
-          if constexpr (ACACHE) {
-            ab = topA[q];
-            at = an[Qd::B(q)];
+        for (int q = 0; q < NGF; ++q) an_b[q] = ACACHE ? an[Qd::B(q)] : -1.0;
+        double fvb[NGF][NV];
+#pragma unroll
+        for (int q = 0; q < NGF; ++q)
+#pragma unroll
+          for (int c = 0; c < NV; ++c) fvb[q][c] = fv[Qd::B(q)][c];
+        yface(topPrev, fvb, topA, an_b, r, Gy, Ey);
+      } else {
+#pragma unroll
+        for (int c = 0; c < NV; ++c) {
+          Gy[c] = 0.0;
+          Ey[c] = 0.0;
+        }
+      }
+      if constexpr (PAIR) {
+        if (upper && r == ym) {  // row ym's bottom values wait for the lower half
+#pragma unroll
+          for (int q = 0; q < NGF; ++q) {
+#pragma unroll
+            for (int c = 0; c < NV; ++c) X_(1, q, c) = fv[Qd::B(q)][c];
+            X_(1, q, NV) = ACACHE ? an[Qd::B(q)] : -1.0;
           }
-          const double s = riemann<S, PRIM, 1>(eos, topPrev[q], fv[Qd::B(q)], Fq, Dq, b, ab, at);
-          if (owned) {
-            if (s > smax) smax = s;
-            if (b.kind) {
-              const long key = (long)ny * (nx + 1) * NGF + ((long)r * nx + col) * NGF + q;
-              const unsigned long long pk = pack_err(seq, key, b.kind, b.val, I->keybits);
-              if (pk < err) err = pk;
-            }
-          }
+        }
+        if (upper && r == ym + 1) {  // so does the assembly of row ym
 #pragma unroll
           for (int c = 0; c < NV; ++c) {
-            Gy[c] = Gy[c] + WF * Fq[c];
-            if (S::DNZ[c]) Ey[c] = Ey[c] + WF * Dq[c];
+            D_(0, c) = Gy[c];
+            D_(1, c) = Ey[c];
+            D_(2, c) = xpartPrev[c];
+            D_(3, c) = ncpPrev[c];
           }
         }
       }
 
       // ---- assembly + stage update of cell (col, r-1) --------------------
-      if (doAsm && owned && mode != HYPE_MODE_PRIME) {
-        const size_t idx = ((size_t)(r - 1) * nx + col) * NV;
-        double rhs[NV];
-#pragma unroll
-        for (int c = 0; c < NV; ++c) {
-          const double Y = Gy[c] - Gprev[c] + 0.5 * (Ey[c] + Eprev[c]);
-          const double t = xpartPrev[c] - r1_h * Y;
-          rhs[c] = ((S::HAS_NCP && S::DNZ[c]) ? ncpPrev[c] : 0.0) + t;
-        }
-        if (mode == HYPE_MODE_RHS) {
-          double* out = L.rhs_out ? L.rhs_out : I->buf[1];
-#pragma unroll
-          for (int c = 0; c < NV; ++c) out[idx + c] = rhs[c];
-        } else {
-          double* out = I->buf[sd.out];
-          if (sd.op == 0) {
-#pragma unroll
-            for (int c = 0; c < NV; ++c) out[idx + c] = Uin[idx + c] + cdt * rhs[c];
-          } else if (sd.op == 2) {
-            // RK104 after its fifth stage: w0 += dt/6 F (VecAXPY); w1 =
-            // 1/25 U + 9/25 w0 (VecAXPBYPCZ, c = 0) into U; w0 = 15 w1 - 5 w0
-            // (VecAXPBY) into A
-            double* A = I->buf[1];
-#pragma unroll
-            for (int c = 0; c < NV; ++c) {
-              const double w0 = Uin[idx + c] + cdt * rhs[c];
-              const double w1 = sd.a * out[idx + c] + sd.g * w0;
-              out[idx + c] = w1;
-              A[idx + c] = 15.0 * w1 + -5.0 * w0;
-            }
-          } else {
-            const double* X = I->buf[sd.xb];
-            const double* Z = I->buf[sd.zb];
-#pragma unroll
-            for (int c = 0; c < NV; ++c) out[idx + c] = sd.a * X[idx + c] + cdt * rhs[c] + sd.g * Z[idx + c];
-          }
-        }
-      }
+      if (doAsm) assemble(r - 1, Gy, Gprev, Ey, Eprev, xpartPrev, ncpPrev);
 
       // ---- carry -----------------------------------------------------------
 #pragma unroll
@@ -533,7 +629,48 @@ struct Tile {
         for (int q = 0; q < NGF; ++q) topA[q] = an[Qd::T(q)];
       }
 
-      __syncthreads();  // the new row is visible to the next iteration
+      sync_window();  // the new row is visible to the next iteration
+    }
+
+    // ---- the two halves of a split tile meet at face ym-1/2 -----------------
+    if constexpr (PAIR) {
+      if (split) {
+        if (!upper) {
+#pragma unroll
+          for (int q = 0; q < NGF; ++q) {
+#pragma unroll
+            for (int c = 0; c < NV; ++c) X_(0, q, c) = topPrev[q][c];
+            X_(0, q, NV) = topA[q];
+          }
+        }
+        __syncthreads();
+        double lo[NGF][NV], hi[NGF][NV], alo[NGF], ahi[NGF];
+#pragma unroll
+        for (int q = 0; q < NGF; ++q) {
+#pragma unroll
+          for (int c = 0; c < NV; ++c) {
+            lo[q][c] = X_(0, q, c);
+            hi[q][c] = X_(1, q, c);
+          }
+          alo[q] = X_(0, q, NV);
+          ahi[q] = X_(1, q, NV);
+        }
+        double Gm[NV], Em[NV];
+        yface(lo, hi, alo, ahi, ym, Gm, Em);
+        if (!upper) {
+          assemble(ym - 1, Gm, Gprev, Em, Eprev, xpartPrev, ncpPrev);
+        } else {
+          double Gt[NV], Et[NV], xp[NV], nc[NV];
+#pragma unroll
+          for (int c = 0; c < NV; ++c) {
+            Gt[c] = D_(0, c);
+            Et[c] = D_(1, c);
+            xp[c] = D_(2, c);
+            nc[c] = D_(3, c);
+          }
+          assemble(ym, Gt, Gm, Et, Em, xp, nc);
+        }
+      }
     }
 
     // ---- CFL maximum and error word ----------------------------------------
@@ -556,24 +693,29 @@ struct Tile {
 };
 
 template <class S, bool PRIM, int N>
-__global__ __launch_bounds__(HYPE_WAVE) __attribute__((amdgpu_waves_per_eu(S::WAVES_PER_EU))) void stage_kernel(
-    hype_launch L) {
-  __shared__ double win[NSLOT * S::NV * COLS];
-  const int lane = threadIdx.x;
+__global__ __launch_bounds__(S::PAIR ? 2 * HYPE_WAVE : HYPE_WAVE)
+    __attribute__((amdgpu_waves_per_eu(S::WAVES_PER_EU))) void stage_kernel(hype_launch L) {
+  using TT = Tile<S, PRIM, N>;
+  __shared__ double lds[TT::LDS];
+  __shared__ int next_tile;
+  const int lane = (int)threadIdx.x % HYPE_WAVE;
+  const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x / HYPE_WAVE);
   const int ntiles = L.ntiles_dev ? *L.ntiles_dev : L.ntiles;
   if (L.grid <= 0) {
-    if ((int)blockIdx.x < ntiles) Tile<S, PRIM, N>::run(L, blockIdx.x, win, lane);
+    if ((int)blockIdx.x < ntiles) TT::run(L, blockIdx.x, lds, lane, wave);
     return;
   }
   for (;;) {
-    unsigned t = 0;
-    if (lane == 0) t = atomicAdd(&L.queue[0], 1u);
-    const int tile = __builtin_amdgcn_readfirstlane((int)__shfl(t, 0, HYPE_WAVE));
-    if (tile >= ntiles) break;
-    Tile<S, PRIM, N>::run(L, tile, win, lane);
+    // one queue pull per workgroup; the second barrier keeps the next pull
+    // from overwriting next_tile before every wave has read it
+    if (threadIdx.x == 0) next_tile = (int)atomicAdd(&L.queue[0], 1u);
     __syncthreads();
+    const int tile = __builtin_amdgcn_readfirstlane(next_tile);
+    __syncthreads();
+    if (tile >= ntiles) break;
+    TT::run(L, tile, lds, lane, wave);
   }
-  if (lane == 0) {
+  if (threadIdx.x == 0) {
     __threadfence();
     const unsigned fin = atomicAdd(&L.queue[1], 1u);
     if (fin == gridDim.x - 1) {
@@ -586,7 +728,7 @@ __global__ __launch_bounds__(HYPE_WAVE) __attribute__((amdgpu_waves_per_eu(S::WA
 template <class S, bool PRIM, int N>
 static hipError_t launch_variant(const hype_launch& L, hipStream_t s) {
   const int grid = L.grid > 0 ? L.grid : L.ntiles;  // device-resident counts need grid > 0
-  hipLaunchKernelGGL((stage_kernel<S, PRIM, N>), dim3(grid), dim3(HYPE_WAVE), 0, s, L);
+  hipLaunchKernelGGL((stage_kernel<S, PRIM, N>), dim3(grid), dim3(HYPE_WAVE * Tile<S, PRIM, N>::WAVES), 0, s, L);
   return hipGetLastError();
 }
 
@@ -604,7 +746,8 @@ static hipError_t launch_sys(const hype_launch& L, hipStream_t s) {
 template <class S, bool PRIM, int N>
 static int occupancy_variant() {
   int blocks = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, stage_kernel<S, PRIM, N>, HYPE_WAVE, 0) != hipSuccess)
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, stage_kernel<S, PRIM, N>, HYPE_WAVE * Tile<S, PRIM, N>::WAVES,
+                                                   0) != hipSuccess)
     return 0;
   return blocks;
 }
@@ -617,6 +760,15 @@ static int occupancy_sys(const hype_variant& v) {
     case 1: return p ? occupancy_variant<S, true, 1>() : occupancy_variant<S, false, 1>();
     case 3: return p ? occupancy_variant<S, true, 3>() : occupancy_variant<S, false, 3>();
     default: return p ? occupancy_variant<S, true, 2>() : occupancy_variant<S, false, 2>();
+  }
+}
+
+// waves per workgroup of the stage kernel of a variant
+int block_waves(const hype_variant& v) {
+  switch (v.system) {
+    case HYPE_EULER: return Euler::PAIR ? 2 : 1;
+    case HYPE_KAPILA: return Kapila::PAIR ? 2 : 1;
+    default: return BN::PAIR ? 2 : 1;
   }
 }
 

@@ -66,34 +66,42 @@ int hype_strips(int nx) { return (nx + HYPE_TILE_W - 1) / HYPE_TILE_W; }
 /* Tile-shape model.  A wave sweeps its strip one row per iteration with all
  * 64 lanes busy whatever the strip width, so a tile of h owned rows costs
  * about (h + 2 + c0) row-iterations (two ring rows, c0 for the 5-row
- * prologue).  The kernel is bound by FP64 issue: waves that share a SIMD
- * share its issue slots, gaining only latency hiding (a measured 1.5x for
- * two waves).  With w = ceil(T / simds) waves per SIMD and `occ` resident
- * waves per SIMD, a launch of T tiles therefore takes about
- *   (h + 2 + c0) * w / gain(w)                      if w <= occ,
- *   (h + 2 + c0) * ceil(w / occ) * occ / gain(occ)   otherwise.
- * Returns the number of row segments minimising that. */
+ * prologue).  Where the two halves of a tile run on two waves (pair), each
+ * wave sweeps about h/2 rows plus one ring row, and the two meet at one
+ * face (a quarter of an iteration).  The kernel is bound by FP64 issue:
+ * waves that share a SIMD share its issue slots, gaining only latency
+ * hiding (a measured 1.5x for two waves).  With w = ceil(waves / simds)
+ * waves per SIMD and `occ` resident waves per SIMD, a launch therefore takes
+ * about
+ *   per * w / gain(w)                      if w <= occ,
+ *   per * ceil(w / occ) * occ / gain(occ)   otherwise,
+ * with per the row-iterations of one wave. */
 static double occ_gain(long w) { return w >= 2 ? 1.5 : 1.0; }
 
-double hype_tile_cost(long tiles, int hmax, int simds, int occ)
+double hype_tile_rows(int h, int pair)
 {
     const double c0 = 1.0;
-    const double per = hmax + 2.0 + c0;
+    if (pair && h >= 2) return (double)((h + 1) / 2) + 1.0 + c0 + 0.25;
+    return h + 2.0 + c0;
+}
+
+double hype_tile_cost(long waves, double per, int simds, int occ)
+{
     long w;
     if (simds < 1) simds = 1;
     if (occ < 1) occ = 1;
-    w = (tiles + simds - 1) / simds;
+    w = (waves + simds - 1) / simds;
     if (w <= occ) return per * (double)w / occ_gain(w);
     return per * (double)((w + occ - 1) / occ) * occ / occ_gain(occ);
 }
 
-int hype_pick_segments(int nx, int ny, int simds, int occ, int extra_tiles)
+int hype_pick_segments(int nx, int ny, int simds, int occ, int extra_waves, int pair)
 {
     int nsx = hype_strips(nx), k, best = 1;
     double best_cost = 1e300;
     for (k = 1; k <= ny; ++k) {
-        long tiles = (long)nsx * k + extra_tiles;
-        double cost = hype_tile_cost(tiles, (ny + k - 1) / k, simds, occ);
+        long waves = (long)nsx * k * (pair ? 2 : 1) + extra_waves;
+        double cost = hype_tile_cost(waves, hype_tile_rows((ny + k - 1) / k, pair), simds, occ);
         if (cost < best_cost - 1e-9) {
             best_cost = cost;
             best = k;

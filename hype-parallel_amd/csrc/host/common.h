@@ -18,10 +18,13 @@ int hype_check_problem(const hype_problem *p, int *nvar);
 int hype_strips(int nx);
 
 /* number of row segments for a grid on a device with `simds` SIMDs and
- * `occ` resident waves of the kernel per SIMD (tile-shape model in common.c) */
-int hype_pick_segments(int nx, int ny, int simds, int occ, int extra_tiles);
-/* modelled time of a launch of `tiles` tiles at most hmax rows tall */
-double hype_tile_cost(long tiles, int hmax, int simds, int occ);
+ * `occ` resident waves of the kernel per SIMD; pair: each tile runs on two
+ * waves, one per half (tile-shape model in common.c) */
+int hype_pick_segments(int nx, int ny, int simds, int occ, int extra_waves, int pair);
+/* row-iterations of one wave for a tile of h rows */
+double hype_tile_rows(int h, int pair);
+/* modelled time of a launch of `waves` waves of `per` row-iterations each */
+double hype_tile_cost(long waves, double per, int simds, int occ);
 
 /* build the tile list of one instance (nsx strips x nsy segments, near-equal
  * splits) into tiles (capacity cap); returns the number of tiles, the first

@@ -16,7 +16,7 @@
 
 struct hype_ctx {
     hype_problem P;
-    int nv, nstages, nsy, ntiles, grid, slots;
+    int nv, nstages, nsy, ntiles, grid, slots, bwaves;
     size_t ncell, nbytes;
     void *stream;
     double *buf[3];         /* U, A, B (device) */
@@ -75,13 +75,14 @@ int hype_ctx_create(const hype_problem *p, hype_ctx **out)
         v.prim = p->recons_prim;
         v.order = p->order;
         v.fma = (p->options & HYPE_OPT_FMA) != 0;
-        c->slots = hyped_num_cus() * hyped_blocks_per_cu(&v);
+        c->bwaves = hyped_block_waves(&v);
+        c->slots = hyped_num_cus() * hyped_blocks_per_cu(&v) * c->bwaves; /* resident waves */
         if (c->slots < 1) c->slots = 1024;
     }
     {
         int ncu = hyped_num_cus();
         if (ncu < 1) ncu = 256;
-        c->nsy = hype_pick_segments(p->nx, p->ny, 4 * ncu, (c->slots + 4 * ncu - 1) / (4 * ncu), 0);
+        c->nsy = hype_pick_segments(p->nx, p->ny, 4 * ncu, (c->slots + 4 * ncu - 1) / (4 * ncu), 0, c->bwaves > 1);
     }
     c->ntiles = hype_strips(p->nx) * c->nsy;
     c->grid = 0;

@@ -89,6 +89,7 @@ static int find_group(hype_engine *E, const hype_variant *v)
             return -1;
         G->grid = E->ncu * hyped_blocks_per_cu(v);
         if (G->grid < 1) G->grid = E->ncu;
+        G->bwaves = hyped_block_waves(v);
         /* a new branch: the step graph must be captured again */
         if (E->graph) {
             hyped_graph_destroy(E->graph);
@@ -192,22 +193,25 @@ int eng_pick_rows(hype_engine *E)
     int h, best = 16, g, simds = 4 * E->ncu, occ = 1 << 30;
     double best_cost = 1e300;
     for (g = 0; g < E->ng; ++g) {
-        int o = E->g[g].grid / simds;
+        int o = E->g[g].grid * E->g[g].bwaves / simds;
         if (o < occ) occ = o;
     }
     if (occ < 1 || occ == 1 << 30) occ = 1;
     for (h = 1; h <= 128; ++h) {
-        long tiles = 0;
-        int s, hmax = 0;
+        long waves = 0;
+        int s;
+        double per = 0.0;
         for (s = 0; s < E->cap; ++s)
             if (E->used[s] && E->run[s]) {
-                int ny = E->P[s].ny, nsy = (ny + h - 1) / h, ht = (ny + nsy - 1) / nsy;
-                tiles += (long)hype_strips(E->P[s].nx) * nsy;
-                if (ht > hmax) hmax = ht;
+                const int ny = E->P[s].ny, nsy = (ny + h - 1) / h, ht = (ny + nsy - 1) / nsy;
+                const int bw = E->g[E->group_of[s]].bwaves;
+                const double pw = hype_tile_rows(ht, bw > 1);
+                waves += (long)hype_strips(E->P[s].nx) * nsy * bw;
+                if (pw > per) per = pw;
             }
-        if (tiles == 0) break;
+        if (waves == 0) break;
         {
-            const double cost = hype_tile_cost(tiles, hmax, simds, occ);
+            const double cost = hype_tile_cost(waves, per, simds, occ);
             if (cost < best_cost - 1e-9) {
                 best_cost = cost;
                 best = h;

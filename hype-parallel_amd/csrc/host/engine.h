@@ -28,6 +28,7 @@ typedef struct {
     int tile_cap;
     int ntiles, nprime;     /* host mirrors */
     int grid;               /* persistent workgroups */
+    int bwaves;             /* waves per workgroup (2: tile halves on two waves) */
     void *stream;           /* graph branch */
     void *ev_done;
 } eng_group;

@@ -126,6 +126,9 @@ int hyped_mem_info(size_t *free_bytes, size_t *total_bytes);
 int hyped_num_cus(void);
 /* resident workgroups per CU of the stage kernel of a variant */
 int hyped_blocks_per_cu(const hype_variant *v);
+/* waves per workgroup of that kernel: 2 where the two halves of each tile
+ * run on their own waves (see the stage kernel), else 1 */
+int hyped_block_waves(const hype_variant *v);
 const char *hyped_last_error(void);
 
 /* HIP events on a stream */
