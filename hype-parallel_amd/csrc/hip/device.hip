@@ -13,7 +13,7 @@
 namespace hype {
 hipError_t launch_stage(const hype_launch& L, hipStream_t s);
 int blocks_per_cu(const hype_variant& v);
-int block_waves(const hype_variant& v);
+int pair_modes(int system);
 hipError_t launch_cons2prim(int system, const double* U, double* W, long n, Eos e, hipStream_t s);
 }  // namespace hype
 namespace hype_fc {  // the same kernels compiled with FMA contraction (sweep_fc.hip)
@@ -154,7 +154,9 @@ int hyped_num_cus(void) {
 
 int hyped_blocks_per_cu(const hype_variant* v) { return v->fma ? hype_fc::blocks_per_cu(*v) : hype::blocks_per_cu(*v); }
 
-int hyped_block_waves(const hype_variant* v) { return hype::block_waves(*v); }  // same in both builds
+int hyped_block_waves(const hype_variant* v) { return v->pair ? 2 : 1; }
+
+int hyped_pair_modes(int system) { return hype::pair_modes(system); }  // the same in both builds
 
 int hyped_graph_begin(void* s) {
   HCHECK(hipStreamBeginCapture((hipStream_t)s, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");

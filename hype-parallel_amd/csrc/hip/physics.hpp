@@ -50,10 +50,12 @@ struct Nrm {
 struct Euler {
   static constexpr int NV = 4;
   static constexpr int WAVES_PER_EU = 2;  // register budget of the stage kernel: 256 per lane
-  // one wave per workgroup: on the 256^2 grids this system is benchmarked on,
-  // split tiles give fewer, two-wave workgroups, and the dispatcher then puts
-  // two waves on one SIMD while others idle (measured 17 % slower)
-  static constexpr bool PAIR = false;
+  // tile layouts the stage kernel is built for (bit 0: one wave per tile,
+  // bit 1: split tiles on two-wave workgroups); the host picks per problem.
+  // Split tiles run one wave per SIMD, one-wave tiles two: small grids gain
+  // from the shorter sweeps (256^2: 540 -> 601 Mcell-updates/s), large ones
+  // from the second wave's latency hiding
+  static constexpr int PAIR_MODES = 3;
   // no sound speed is shared between the admissibility check and the flux
   static constexpr bool HAS_ACACHE = false;
   static constexpr int REFX[2] = {1, -1};  // reflective x-wall negates comp 1
@@ -153,9 +155,9 @@ struct Euler {
 struct Kapila {
   static constexpr int NV = 6;
   static constexpr int WAVES_PER_EU = 1;
-  // two waves per workgroup share each tile: one reconstructed ring row per
-  // wave saved (1024 x 512 shock-bubble: 952 -> 987 Mcell-updates/s)
-  static constexpr bool PAIR = true;
+  // split tiles only: one reconstructed ring row per wave saved at the same
+  // one wave per SIMD (1024 x 512 shock-bubble: 952 -> 985 Mcell-updates/s)
+  static constexpr int PAIR_MODES = 2;
   // the admissibility check at a node computes the a_sq the primitive flux
   // at that point needs (kapila_5eqn.c:347-349 and 468-470 are the same
   // expression): the stage kernel hands it over instead of recomputing
@@ -426,9 +428,9 @@ struct Kapila {
 struct BN {
   static constexpr int NV = 9;
   static constexpr int WAVES_PER_EU = 1;
-  // one wave per workgroup: two windows of nine components would halve the
+  // one wave per tile only: two windows of nine components would halve the
   // workgroups an LDS of 160 KB holds
-  static constexpr bool PAIR = false;
+  static constexpr int PAIR_MODES = 1;
   // no sound speed is shared between the admissibility check and the flux
   static constexpr bool HAS_ACACHE = false;
   static constexpr int REFX[2] = {1, 5};

@@ -21,7 +21,7 @@
 //     fused; the stage output goes straight to HBM, and the wave's largest
 //     wave speed feeds the device-wide CFL maximum (:617-623).
 //
-// Systems with S::PAIR run two waves per workgroup.  A tile of two or more
+// With split tiles (PR) the kernel runs two waves per workgroup.  A tile of two or more
 // rows is split at its middle row ym; both waves march up.  The lower wave
 // starts on the ring row below the tile as above and stops at row ym-1.
 // The upper wave starts at ym with no ring row below it.  It sets aside row
@@ -138,12 +138,12 @@ __device__ __forceinline__ void wave_sync() {
 constexpr int NSLOT = 5;
 constexpr int COLS = HYPE_WAVE + 4;
 
-template <class S, bool PRIM, int N>
+template <class S, bool PRIM, int N, bool PR>
 struct Tile {
   static constexpr int NV = S::NV;
   using Qd = Quad<N>;
   static constexpr int NDOF = Qd::NDOF, NGF = Qd::NGF, NGV = Qd::NGV, NNODE = Qd::NNODE;
-  static constexpr bool PAIR = S::PAIR;
+  static constexpr bool PAIR = PR;
   static constexpr int WAVES = PAIR ? 2 : 1;
   // the admissibility check's a_sq at each node goes on to the flux there
   static constexpr bool ACACHE = PRIM && S::HAS_ACACHE && N == 2;
@@ -692,10 +692,21 @@ struct Tile {
   }
 };
 
-template <class S, bool PRIM, int N>
-__global__ __launch_bounds__(S::PAIR ? 2 * HYPE_WAVE : HYPE_WAVE)
-    __attribute__((amdgpu_waves_per_eu(S::WAVES_PER_EU))) void stage_kernel(hype_launch L) {
-  using TT = Tile<S, PRIM, N>;
+// Split tiles run one wave per SIMD: the two waves of a workgroup must not
+// share a SIMD's issue slots while other SIMDs idle.  A system whose kernel
+// would fit two waves per SIMD gets its accumulation registers reserved (an
+// empty asm statement that clobbers a255), which leaves room for one wave.
+template <class S, bool PR>
+struct Occ {
+  static constexpr bool ONE_PER_SIMD = PR && S::WAVES_PER_EU > 1;
+  static constexpr int WAVES_PER_EU = PR ? 1 : S::WAVES_PER_EU;
+};
+
+template <class S, bool PRIM, int N, bool PR>
+__global__ __launch_bounds__(PR ? 2 * HYPE_WAVE : HYPE_WAVE)
+    __attribute__((amdgpu_waves_per_eu(Occ<S, PR>::WAVES_PER_EU))) void stage_kernel(hype_launch L) {
+  using TT = Tile<S, PRIM, N, PR>;
+  if constexpr (Occ<S, PR>::ONE_PER_SIMD) asm volatile("" ::: "a255");
   __shared__ double lds[TT::LDS];
   __shared__ int next_tile;
   const int lane = (int)threadIdx.x % HYPE_WAVE;
@@ -725,50 +736,73 @@ __global__ __launch_bounds__(S::PAIR ? 2 * HYPE_WAVE : HYPE_WAVE)
   }
 }
 
-template <class S, bool PRIM, int N>
+template <class S, bool PRIM, int N, bool PR>
 static hipError_t launch_variant(const hype_launch& L, hipStream_t s) {
   const int grid = L.grid > 0 ? L.grid : L.ntiles;  // device-resident counts need grid > 0
-  hipLaunchKernelGGL((stage_kernel<S, PRIM, N>), dim3(grid), dim3(HYPE_WAVE * Tile<S, PRIM, N>::WAVES), 0, s, L);
+  hipLaunchKernelGGL((stage_kernel<S, PRIM, N, PR>), dim3(grid), dim3(HYPE_WAVE * Tile<S, PRIM, N, PR>::WAVES), 0, s,
+                     L);
   return hipGetLastError();
+}
+
+// the tile layout the variant asks for, if the system is built for it
+template <class S, bool PRIM, int N>
+static hipError_t launch_layout(const hype_launch& L, hipStream_t s) {
+  if (L.v.pair) {
+    if constexpr ((S::PAIR_MODES & 2) != 0) return launch_variant<S, PRIM, N, true>(L, s);
+  } else {
+    if constexpr ((S::PAIR_MODES & 1) != 0) return launch_variant<S, PRIM, N, false>(L, s);
+  }
+  return hipErrorInvalidValue;
 }
 
 template <class S>
 static hipError_t launch_sys(const hype_launch& L, hipStream_t s) {
   const bool p = L.v.prim != 0;
   switch (L.v.order) {
-    case 0: return p ? launch_variant<S, true, 0>(L, s) : launch_variant<S, false, 0>(L, s);
-    case 1: return p ? launch_variant<S, true, 1>(L, s) : launch_variant<S, false, 1>(L, s);
-    case 3: return p ? launch_variant<S, true, 3>(L, s) : launch_variant<S, false, 3>(L, s);
-    default: return p ? launch_variant<S, true, 2>(L, s) : launch_variant<S, false, 2>(L, s);
+    case 0: return p ? launch_layout<S, true, 0>(L, s) : launch_layout<S, false, 0>(L, s);
+    case 1: return p ? launch_layout<S, true, 1>(L, s) : launch_layout<S, false, 1>(L, s);
+    case 3: return p ? launch_layout<S, true, 3>(L, s) : launch_layout<S, false, 3>(L, s);
+    default: return p ? launch_layout<S, true, 2>(L, s) : launch_layout<S, false, 2>(L, s);
   }
 }
 
-template <class S, bool PRIM, int N>
+template <class S, bool PRIM, int N, bool PR>
 static int occupancy_variant() {
   int blocks = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, stage_kernel<S, PRIM, N>, HYPE_WAVE * Tile<S, PRIM, N>::WAVES,
-                                                   0) != hipSuccess)
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, stage_kernel<S, PRIM, N, PR>,
+                                                   HYPE_WAVE * Tile<S, PRIM, N, PR>::WAVES, 0) != hipSuccess)
     return 0;
   return blocks;
+}
+
+template <class S, bool PRIM, int N>
+static int occupancy_layout(const hype_variant& v) {
+  if (v.pair) {
+    if constexpr ((S::PAIR_MODES & 2) != 0) return occupancy_variant<S, PRIM, N, true>();
+  } else {
+    if constexpr ((S::PAIR_MODES & 1) != 0) return occupancy_variant<S, PRIM, N, false>();
+  }
+  return 0;
 }
 
 template <class S>
 static int occupancy_sys(const hype_variant& v) {
   const bool p = v.prim != 0;
   switch (v.order) {
-    case 0: return p ? occupancy_variant<S, true, 0>() : occupancy_variant<S, false, 0>();
-    case 1: return p ? occupancy_variant<S, true, 1>() : occupancy_variant<S, false, 1>();
-    case 3: return p ? occupancy_variant<S, true, 3>() : occupancy_variant<S, false, 3>();
-    default: return p ? occupancy_variant<S, true, 2>() : occupancy_variant<S, false, 2>();
+    case 0: return p ? occupancy_layout<S, true, 0>(v) : occupancy_layout<S, false, 0>(v);
+    case 1: return p ? occupancy_layout<S, true, 1>(v) : occupancy_layout<S, false, 1>(v);
+    case 3: return p ? occupancy_layout<S, true, 3>(v) : occupancy_layout<S, false, 3>(v);
+    default: return p ? occupancy_layout<S, true, 2>(v) : occupancy_layout<S, false, 2>(v);
   }
 }
 
-// waves per workgroup of the stage kernel of a variant
-int block_waves(const hype_variant& v) {
-  switch (v.system) {
-    case HYPE_EULER: return Euler::PAIR ? 2 : 1;
-    case HYPE_KAPILA: return Kapila::PAIR ? 2 : 1;
-    default: return BN::PAIR ? 2 : 1;
+// tile layouts a system's stage kernel is built for (bit 0: one wave per
+// tile, bit 1: split tiles)
+int pair_modes(int system) {
+  switch (system) {
+    case HYPE_EULER: return Euler::PAIR_MODES;
+    case HYPE_KAPILA: return Kapila::PAIR_MODES;
+    default: return BN::PAIR_MODES;
   }
 }
 

@@ -28,7 +28,8 @@ int hype_check_problem(const hype_problem *p, int *nvar)
         return HYPE_E_UNSUPPORTED;
     if (!(p->x_max > p->x_min) || !(p->y_max > p->y_min)) return HYPE_E_INVALID;
     if (!(p->cfl > 0.0)) return HYPE_E_INVALID;
-    if (p->options & ~HYPE_OPT_FMA) return HYPE_E_INVALID;
+    if (p->options & ~(HYPE_OPT_FMA | HYPE_OPT_ONE_WAVE_TILES | HYPE_OPT_SPLIT_TILES)) return HYPE_E_INVALID;
+    if ((p->options & HYPE_OPT_ONE_WAVE_TILES) && (p->options & HYPE_OPT_SPLIT_TILES)) return HYPE_E_INVALID;
     for (s = 0; s < 4; ++s) {
         if (p->bc[s] == HYPE_BC_WALL || p->bc[s] == HYPE_BC_INLET) return HYPE_E_UNSUPPORTED;
         if (p->bc[s] != HYPE_BC_PERIODIC && p->bc[s] != HYPE_BC_REFLECTIVE &&
@@ -151,6 +152,16 @@ void hype_decode_error(unsigned long long word, int nx, int ny, int ngf,
     if (kind < 1 || kind > 7) kind = 0;
     snprintf(head, sizeof(head), what[kind], (double)fv);
     snprintf(msg, cap, "%s\nAt x = %f, y = %f", head, *x, *y);
+}
+
+/* tile layouts (bit 0: one wave per tile, bit 1: split tiles) a problem may
+ * use: those the system's kernel is built for, narrowed by the layout bits
+ * of its options */
+int hype_layouts(const hype_problem *p, int built)
+{
+    if (p->options & HYPE_OPT_ONE_WAVE_TILES) return built & 1;
+    if (p->options & HYPE_OPT_SPLIT_TILES) return built & 2;
+    return built;
 }
 
 void hype_fill_inst(hype_inst *I, const hype_problem *p, int nsy)

@@ -21,6 +21,9 @@ int hype_strips(int nx);
  * `occ` resident waves of the kernel per SIMD; pair: each tile runs on two
  * waves, one per half (tile-shape model in common.c) */
 int hype_pick_segments(int nx, int ny, int simds, int occ, int extra_waves, int pair);
+/* tile layouts a problem may use (bit 0: one wave per tile, bit 1: split
+ * tiles), given those its system's kernel is built for */
+int hype_layouts(const hype_problem *p, int built);
 /* row-iterations of one wave for a tile of h rows */
 double hype_tile_rows(int h, int pair);
 /* modelled time of a launch of `waves` waves of `per` row-iterations each */

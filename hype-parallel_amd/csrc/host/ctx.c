@@ -16,7 +16,8 @@
 
 struct hype_ctx {
     hype_problem P;
-    int nv, nstages, nsy, ntiles, grid, slots, bwaves;
+    int nv, nstages, nsy, ntiles, grid, slots;
+    int pair; /* 1: split tiles on two-wave workgroups */
     size_t ncell, nbytes;
     void *stream;
     double *buf[3];         /* U, A, B (device) */
@@ -70,19 +71,41 @@ int hype_ctx_create(const hype_problem *p, hype_ctx **out)
     c->ncell = (size_t)p->nx * p->ny;
     c->nbytes = c->ncell * nv * sizeof(double);
     {
-        hype_variant v;
-        v.system = p->system;
-        v.prim = p->recons_prim;
-        v.order = p->order;
-        v.fma = (p->options & HYPE_OPT_FMA) != 0;
-        c->bwaves = hyped_block_waves(&v);
-        c->slots = hyped_num_cus() * hyped_blocks_per_cu(&v) * c->bwaves; /* resident waves */
-        if (c->slots < 1) c->slots = 1024;
-    }
-    {
-        int ncu = hyped_num_cus();
+        /* tile layout and row split: the tile-shape model's cheapest among the
+         * layouts the system's kernel is built for (one wave per tile, or split
+         * tiles on two-wave workgroups) */
+        const int modes = hype_layouts(p, hyped_pair_modes(p->system));
+        int ncu = hyped_num_cus(), pr;
+        double best = 1e300;
         if (ncu < 1) ncu = 256;
-        c->nsy = hype_pick_segments(p->nx, p->ny, 4 * ncu, (c->slots + 4 * ncu - 1) / (4 * ncu), 0, c->bwaves > 1);
+        c->pair = -1;
+        for (pr = 0; pr <= 1; ++pr) {
+            hype_variant v;
+            int slots, occ, nsy;
+            double cost;
+            if (!(modes & (1 << pr))) continue;
+            v.system = p->system;
+            v.prim = p->recons_prim;
+            v.order = p->order;
+            v.fma = (p->options & HYPE_OPT_FMA) != 0;
+            v.pair = pr;
+            slots = ncu * hyped_blocks_per_cu(&v) * hyped_block_waves(&v); /* resident waves */
+            if (slots < 1) slots = 4 * ncu;
+            occ = (slots + 4 * ncu - 1) / (4 * ncu);
+            nsy = hype_pick_segments(p->nx, p->ny, 4 * ncu, occ, 0, pr);
+            cost = hype_tile_cost((long)hype_strips(p->nx) * nsy * hyped_block_waves(&v),
+                                  hype_tile_rows((p->ny + nsy - 1) / nsy, pr), 4 * ncu, occ);
+            if (cost < best - 1e-9) {
+                best = cost;
+                c->pair = pr;
+                c->nsy = nsy;
+                c->slots = slots;
+            }
+        }
+        if (c->pair < 0) {
+            free(c);
+            return HYPE_E_UNSUPPORTED;
+        }
     }
     c->ntiles = hype_strips(p->nx) * c->nsy;
     c->grid = 0;
@@ -131,6 +154,7 @@ static hype_launch base_launch(hype_ctx *c, int mode, int stage)
     L.v.prim = c->P.recons_prim;
     L.v.order = c->P.order;
     L.v.fma = (c->P.options & HYPE_OPT_FMA) != 0;
+    L.v.pair = c->pair;
     L.insts = c->inst_d;
     L.tiles = c->tiles_d;
     L.ntiles = c->ntiles;

@@ -62,7 +62,8 @@ void eng_free(hype_engine *E)
 
 static int same_variant(const hype_variant *a, const hype_variant *b)
 {
-    return a->system == b->system && a->prim == b->prim && a->order == b->order && a->fma == b->fma;
+    return a->system == b->system && a->prim == b->prim && a->order == b->order && a->fma == b->fma &&
+           a->pair == b->pair;
 }
 
 static int find_group(hype_engine *E, const hype_variant *v)
@@ -110,6 +111,14 @@ int eng_attach(hype_engine *E, int slot, const hype_problem *p, double *U, doubl
     v.prim = p->recons_prim;
     v.order = p->order;
     v.fma = (p->options & HYPE_OPT_FMA) != 0;
+    /* split tiles where the system has no other layout (or the problem asks
+     * for them); a batch keeps many waves per SIMD busy, where one-wave tiles
+     * hide latency better */
+    {
+        const int modes = hype_layouts(p, hyped_pair_modes(p->system));
+        if (!modes) return HYPE_E_UNSUPPORTED;
+        v.pair = (modes & 1) ? 0 : 1;
+    }
     g = find_group(E, &v);
     if (g < 0) return HYPE_E_DEVICE;
     E->P[slot] = *p;

@@ -83,6 +83,7 @@ typedef struct hype_tile {
 typedef struct hype_variant {
     int system, prim, order;
     int fma;               /* 1: the FMA-contracted kernel build (HYPE_OPT_FMA) */
+    int pair;              /* 1: each tile split over a two-wave workgroup */
 } hype_variant;
 
 typedef struct hype_launch {
@@ -126,9 +127,12 @@ int hyped_mem_info(size_t *free_bytes, size_t *total_bytes);
 int hyped_num_cus(void);
 /* resident workgroups per CU of the stage kernel of a variant */
 int hyped_blocks_per_cu(const hype_variant *v);
-/* waves per workgroup of that kernel: 2 where the two halves of each tile
- * run on their own waves (see the stage kernel), else 1 */
+/* waves per workgroup of that kernel: 2 with split tiles (the two halves of
+ * each tile on their own waves, see the stage kernel), else 1 */
 int hyped_block_waves(const hype_variant *v);
+/* tile layouts a system's stage kernel is built for: bit 0 one wave per
+ * tile, bit 1 split tiles */
+int hyped_pair_modes(int system);
 const char *hyped_last_error(void);
 
 /* HIP events on a stream */

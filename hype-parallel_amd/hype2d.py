@@ -22,6 +22,8 @@ EULER, KAPILA, BAER_NUNZIATO = 0, 1, 2
 BC_WALL, BC_INLET, BC_PERIODIC, BC_REFLECTIVE, BC_TRANSMISSIVE = range(5)
 QUIRK_BN_P2_IS_PHI = 0x1
 OPT_FMA = 0x1
+OPT_ONE_WAVE_TILES = 0x2  # force one wave per tile
+OPT_SPLIT_TILES = 0x4  # force tiles split over a two-wave workgroup
 MEM_HOST, MEM_DEVICE = 0, 1
 TS_CLIP, TS_GRAPH = 0x1, 0x2
 NVAR = {EULER: 4, KAPILA: 6, BAER_NUNZIATO: 9}

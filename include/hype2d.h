@@ -62,7 +62,7 @@ extern "C" {
  *      (hype.h:203-252) ------------------------------------------------- */
 typedef struct hype_problem {
     int system;            /* HYPE_EULER / HYPE_KAPILA / HYPE_BAER_NUNZIATO */
-    int order;             /* AppCtx.N, 0..2 (reconstruction degree) */
+    int order;             /* AppCtx.N, 0..3 (reconstruction degree) */
     int recons_prim;       /* AppCtx.ReconsPrimitive: 1 RHSFunctionPrim, 0 RHSFunction */
     int nx, ny;            /* AppCtx.N_x, N_y (>= 3) */
     double x_min, x_max;   /* h = (x_max - x_min)/nx for both axes (hype.c:57) */
@@ -83,6 +83,19 @@ typedef struct hype_problem {
  * threshold decision on a value within rounding of it (admissibility, WENO
  * range guard) can then differ.  Default off: bitwise reference arithmetic. */
 #define HYPE_OPT_FMA 0x1u
+
+/* ---- tile layout ----------------------------------------------------------
+ * The stage kernel sweeps tiles of up to 62 columns.  A tile runs either on
+ * one wave, or split at its middle row over the two waves of a workgroup,
+ * which saves one reconstructed ring row per wave.  By default
+ * hype_ctx_create picks the faster layout for the grid (Kapila: split;
+ * Baer-Nunziato: one wave; Euler: split for small grids, one wave when the
+ * grid fills two waves per SIMD), and batches and servers use split tiles
+ * for Kapila and one wave per tile otherwise.  These bits force a layout; one
+ * the system's kernel is not built for is HYPE_E_UNSUPPORTED.  Results are
+ * bitwise the same either way. */
+#define HYPE_OPT_ONE_WAVE_TILES 0x2u
+#define HYPE_OPT_SPLIT_TILES 0x4u
 
 /* number of components of a system (the reference's nVar, hype.h:37) */
 int hype_nvar(int system);

@@ -36,12 +36,13 @@ def rel_err(a, b):
 
 
 def check_rhs(hip, system, prim, order, bc, nx, ny, seed, hrows=None, grid=0, quirks=0, sharp=True,
-              rough=False):
+              rough=False, options=0):
     eos = states.EOS[system]
     U = states.rough_state(system, nx, ny, seed) if rough else states.state(system, nx, ny, seed=seed, sharp=sharp)
     dom = (0.0, 1.0, 0.0, ny / nx)
     p = hip.make_problem(system, nx, ny, order=order, prim=prim, domain=dom, cfl=0.5, bc=bc,
                          quirks=quirks, **eos)
+    p.options = options
     op = orc.problem(system, nx, ny, N=order, prim=prim, domain=dom, cfl=0.5, bc=bc,
                      g1=eos["gamma1"], g2=eos["gamma2"], p1=eos["pi1"], p2=eos["pi2"], quirks=quirks)
     ref = orc.rhs(op, U, want_fallback=True)
@@ -77,6 +78,30 @@ def test_rhs_boundaries(hip, bcname, system):
 @pytest.mark.parametrize("hrows,grid", [(1, 0), (3, 0), (7, 0), (64, 0), (5, 16), (2, 3)])
 def test_rhs_tilings(hip, hrows, grid):
     check_rhs(hip, 1, True, 2, BCS["refl_tb"], 130, 29, seed=3, hrows=hrows, grid=grid)
+
+
+# tile layouts: one wave per tile, or split at the middle row over two waves
+# (tile heights 1 to 5 cover the unsplit one-row tile, even and odd splits;
+# grid > 0 runs the persistent queue with two-wave workgroups)
+LAYOUTS = {"one": 0x2, "split": 0x4}
+
+
+@pytest.mark.parametrize("layout", ["one", "split"])
+@pytest.mark.parametrize("hrows,grid", [(1, 0), (2, 0), (3, 0), (4, 0), (5, 0), (64, 0), (3, 7)])
+@pytest.mark.parametrize("order", [1, 2, 3])
+def test_rhs_euler_tile_layouts(hip, layout, hrows, grid, order):
+    check_rhs(hip, 0, True, order, BCS["mixed"], 130, 29, seed=5, hrows=hrows, grid=grid,
+              options=LAYOUTS[layout])
+
+
+@pytest.mark.parametrize("system,layout", [(1, "one"), (2, "split")])
+def test_rhs_layout_not_built_is_unsupported(hip, system, layout):
+    eos = states.EOS[system]
+    p = hip.make_problem(system, 40, 20, order=2, prim=True, bc=BCS["trans"], **eos)
+    p.options = LAYOUTS[layout]
+    with pytest.raises(hip.HypeError) as e:
+        hip.Context(p)
+    assert e.value.code == hip.E_UNSUPPORTED
 
 
 @pytest.mark.parametrize("nx,ny", [(3, 3), (4, 7), (61, 5), (62, 3), (63, 4), (124, 9), (200, 3)])

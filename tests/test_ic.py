@@ -102,10 +102,14 @@ def test_smooth_vortex_profile():
     assert np.max(np.abs(U[..., 5] - a)) < 0.01  # cell average vs centre value, O(h^2)
 
 
-def test_problem_options_accept_fma_and_reject_unknown_bits():
+def test_problem_options_validated():
     p = H.make_problem(H.EULER, 8, 6, domain=(0, 1, 0, 0.75), gamma1=1.4, fma=True)
     assert p.options == H.OPT_FMA
     H.initial_state(p, H.IC_UNIFORM, [1.0, 0.0, 0.0, 1.0])  # validated, accepted
-    p.options = 0x2
-    with pytest.raises(H.HypeError):
+    for ok in (H.OPT_ONE_WAVE_TILES, H.OPT_SPLIT_TILES | H.OPT_FMA):
+        p.options = ok
         H.initial_state(p, H.IC_UNIFORM, [1.0, 0.0, 0.0, 1.0])
+    for bad in (0x8, H.OPT_ONE_WAVE_TILES | H.OPT_SPLIT_TILES):  # unknown bit; two layouts at once
+        p.options = bad
+        with pytest.raises(H.HypeError):
+            H.initial_state(p, H.IC_UNIFORM, [1.0, 0.0, 0.0, 1.0])
