@@ -146,3 +146,36 @@ def test_autotune_keeps_results(hip):
     for i in range(2):
         ref, _ = solo(hip, probs[i], U0s[i], 5)
         assert np.array_equal(out[i], ref)
+
+
+@pytest.mark.parametrize("tile_rows", [0, 4, 7])
+def test_batch_with_forced_tile_layouts(hip, tile_rows):
+    """Euler instances forced onto one-wave tiles and onto split tiles run as
+    two variants of one batch (two graph branches, persistent queues of one-
+    and two-wave workgroups); each ends bitwise equal to its solo run and to
+    the oracle."""
+    cases = [(0, True, 2, 70, 44, (T, T, R, R), 31, 0x2), (0, True, 2, 70, 44, (T, T, R, R), 31, 0x4),
+             (0, False, 3, 45, 30, (P, P, R, R), 32, 0x4), (0, True, 1, 90, 17, (R, T, P, P), 33, 0x2)]
+    probs, U0s, steps = [], [], []
+    for k, (s, pr, o, nx, ny, bc, seed, opt) in enumerate(cases):
+        p = problem(hip, s, pr, o, nx, ny, bc)
+        p.options = opt
+        probs.append(p)
+        U0s.append(states.state(s, nx, ny, seed=seed))
+        steps.append(6 + max(k - 1, 0))  # the first two: the same problem, the same budget
+    out, st, tiling = run_batch(hip, probs, U0s, np.array(steps, dtype=np.int32), tile_rows=tile_rows)
+    print(f"[batch layouts] tile_rows={tile_rows} tiling={tiling}", flush=True)
+    assert tiling[2] == 4  # (prim, order, layout) differ: four variants
+    for i, p in enumerate(probs):
+        U, s1 = solo(hip, p, U0s[i], steps[i])
+        assert st[i].steps == s1.steps == steps[i]
+        assert st[i].t == s1.t
+        assert np.array_equal(out[i], U), f"instance {i}"
+    # the two layouts of the same problem agree bitwise too
+    assert np.array_equal(out[0], out[1])
+    s, pr, o, nx, ny, bc, seed, _ = cases[0]
+    eos = states.EOS[s]
+    op = orc.problem(s, nx, ny, N=o, prim=pr, domain=(0.0, 1.0, 0.0, ny / nx), cfl=0.5, bc=bc, g1=eos["gamma1"],
+                     g2=eos["gamma2"], p1=eos["pi1"], p2=eos["pi2"])
+    ref = orc.solve(op, U0s[0], steps[0])
+    assert np.array_equal(out[0], ref["U"])
