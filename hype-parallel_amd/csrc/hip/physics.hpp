@@ -56,6 +56,7 @@ struct Euler {
   // from the shorter sweeps (256^2: 540 -> 601 Mcell-updates/s), large ones
   // from the second wave's latency hiding
   static constexpr int PAIR_MODES = 3;
+  static constexpr bool HAS_INLET = false;  // no InletBC in euler.c
   // no sound speed is shared between the admissibility check and the flux
   static constexpr bool HAS_ACACHE = false;
   static constexpr int REFX[2] = {1, -1};  // reflective x-wall negates comp 1
@@ -158,6 +159,8 @@ struct Kapila {
   // split tiles only: one reconstructed ring row per wave saved at the same
   // one wave per SIMD (1024 x 512 shock-bubble: 952 -> 985 Mcell-updates/s)
   static constexpr int PAIR_MODES = 2;
+  // left inlet of kapila_5eqn.c:289-320, 485-513 (tabulated per row by the host)
+  static constexpr bool HAS_INLET = true;
   // the admissibility check at a node computes the a_sq the primitive flux
   // at that point needs (kapila_5eqn.c:347-349 and 468-470 are the same
   // expression): the stage kernel hands it over instead of recomputing
@@ -431,6 +434,7 @@ struct BN {
   // one wave per tile only: two windows of nine components would halve the
   // workgroups an LDS of 160 KB holds
   static constexpr int PAIR_MODES = 1;
+  static constexpr bool HAS_INLET = false;  // no InletBC in baer_nunziato.c
   // no sound speed is shared between the admissibility check and the flux
   static constexpr bool HAS_ACACHE = false;
   static constexpr int REFX[2] = {1, 5};

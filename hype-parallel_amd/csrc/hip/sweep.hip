@@ -138,7 +138,7 @@ __device__ __forceinline__ void wave_sync() {
 constexpr int NSLOT = 5;
 constexpr int COLS = HYPE_WAVE + 4;
 
-template <class S, bool PRIM, int N, bool PR>
+template <class S, bool PRIM, int N, bool PR, bool IN>
 struct Tile {
   static constexpr int NV = S::NV;
   using Qd = Quad<N>;
@@ -279,7 +279,7 @@ struct Tile {
     auto fetch = [&](int rr, int k, double* q) -> int {
       const int cc = min(x0 - 3 + k, nx + 2);
       bool fx = false, fy = false;
-      const int sc = bc_map(cc, nx, bcL, bcR, fx);
+      const int sc = bc_map(cc, nx, bcL, bcR, fx);  // an inlet column maps as periodic; put_inlet replaces it
       const int sr = bc_map(rr, ny, bcB, bcT, fy);
       const double* p = Uin + ((size_t)sr * nx + sc) * NV;
 #pragma unroll
@@ -305,6 +305,31 @@ struct Tile {
 #pragma unroll
       for (int c = 0; c < NV; ++c) W_(s, c, k) = v[c];
     };
+    // left inlet (Kapila, rhs_function_prim.c:135-144): in the strip at the
+    // left edge, the three window columns left of the grid take the state the
+    // host tabulated for the ghost row, once the row is stored.  The y-pass
+    // that follows in the reference (:204-315) maps transmissive and
+    // reflective ghost rows; periodic ones keep their own height.
+    const bool inlet_strip = IN && bcL == HYPE_BC_INLET && x0 == 0;
+    auto put_inlet = [&](int s, int rr) {
+      if constexpr (IN) {
+        if (inlet_strip && lane < 3) {
+          bool fy = false;
+          const int sr = bc_map(rr, ny, bcB, bcT, fy);
+          const bool own = (rr < 0 && bcB == HYPE_BC_PERIODIC) || (rr >= ny && bcT == HYPE_BC_PERIODIC);
+          const double* p = I->inlet + (size_t)((own ? rr : sr) + 3) * NV;
+          double v[NV];
+#pragma unroll
+          for (int c = 0; c < NV; ++c) v[c] = p[c];
+          if (fy) {
+            v[S::REFY[0]] = -v[S::REFY[0]];
+            if (S::REFY[1] >= 0) v[S::REFY[1]] = -v[S::REFY[1]];
+          }
+#pragma unroll
+          for (int c = 0; c < NV; ++c) W_(s, c, lane) = v[c];
+        }
+      }
+    };
 
     // prologue: rows r_first-2 .. r_first+2, all loads in flight before the
     // first store
@@ -321,6 +346,7 @@ struct Tile {
       for (int i = 0; i < 5; ++i) {
         put(slot(rbase + i), lane, qa[i], fa[i]);
         if (lane < 4) put(slot(rbase + i), lane + HYPE_WAVE, qb[i], fb[i]);
+        put_inlet(slot(rbase + i), rbase + i);
       }
     }
     sync_window();
@@ -503,6 +529,7 @@ struct Tile {
         const int sn = slot(r + 3);
         put(sn, lane, qa, fa);
         if (lane < 4) put(sn, lane + HYPE_WAVE, qb, fb);
+        put_inlet(sn, r + 3);
       }
 
       // ---- volume non-conservative product (rhs_function_prim.c:461-479)
@@ -702,10 +729,12 @@ struct Occ {
   static constexpr int WAVES_PER_EU = PR ? 1 : S::WAVES_PER_EU;
 };
 
-template <class S, bool PRIM, int N, bool PR>
+// IN: the build with the left-inlet ghost columns (Kapila problems with an
+// inlet only, so the loop of every other problem is compiled without it)
+template <class S, bool PRIM, int N, bool PR, bool IN>
 __global__ __launch_bounds__(PR ? 2 * HYPE_WAVE : HYPE_WAVE)
     __attribute__((amdgpu_waves_per_eu(Occ<S, PR>::WAVES_PER_EU))) void stage_kernel(hype_launch L) {
-  using TT = Tile<S, PRIM, N, PR>;
+  using TT = Tile<S, PRIM, N, PR, IN>;
   if constexpr (Occ<S, PR>::ONE_PER_SIMD) asm volatile("" ::: "a255");
   __shared__ double lds[TT::LDS];
   __shared__ int next_tile;
@@ -736,21 +765,30 @@ __global__ __launch_bounds__(PR ? 2 * HYPE_WAVE : HYPE_WAVE)
   }
 }
 
-template <class S, bool PRIM, int N, bool PR>
+template <class S, bool PRIM, int N, bool PR, bool IN>
 static hipError_t launch_variant(const hype_launch& L, hipStream_t s) {
   const int grid = L.grid > 0 ? L.grid : L.ntiles;  // device-resident counts need grid > 0
-  hipLaunchKernelGGL((stage_kernel<S, PRIM, N, PR>), dim3(grid), dim3(HYPE_WAVE * Tile<S, PRIM, N, PR>::WAVES), 0, s,
-                     L);
+  hipLaunchKernelGGL((stage_kernel<S, PRIM, N, PR, IN>), dim3(grid), dim3(HYPE_WAVE * Tile<S, PRIM, N, PR, IN>::WAVES), 0,
+                     s, L);
   return hipGetLastError();
+}
+
+template <class S, bool PRIM, int N, bool PR>
+static hipError_t launch_inlet(const hype_launch& L, hipStream_t s) {
+  if (L.v.inlet) {
+    if constexpr (S::HAS_INLET) return launch_variant<S, PRIM, N, PR, true>(L, s);
+    return hipErrorInvalidValue;
+  }
+  return launch_variant<S, PRIM, N, PR, false>(L, s);
 }
 
 // the tile layout the variant asks for, if the system is built for it
 template <class S, bool PRIM, int N>
 static hipError_t launch_layout(const hype_launch& L, hipStream_t s) {
   if (L.v.pair) {
-    if constexpr ((S::PAIR_MODES & 2) != 0) return launch_variant<S, PRIM, N, true>(L, s);
+    if constexpr ((S::PAIR_MODES & 2) != 0) return launch_inlet<S, PRIM, N, true>(L, s);
   } else {
-    if constexpr ((S::PAIR_MODES & 1) != 0) return launch_variant<S, PRIM, N, false>(L, s);
+    if constexpr ((S::PAIR_MODES & 1) != 0) return launch_inlet<S, PRIM, N, false>(L, s);
   }
   return hipErrorInvalidValue;
 }
@@ -766,21 +804,30 @@ static hipError_t launch_sys(const hype_launch& L, hipStream_t s) {
   }
 }
 
-template <class S, bool PRIM, int N, bool PR>
+template <class S, bool PRIM, int N, bool PR, bool IN>
 static int occupancy_variant() {
   int blocks = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, stage_kernel<S, PRIM, N, PR>,
-                                                   HYPE_WAVE * Tile<S, PRIM, N, PR>::WAVES, 0) != hipSuccess)
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, stage_kernel<S, PRIM, N, PR, IN>,
+                                                   HYPE_WAVE * Tile<S, PRIM, N, PR, IN>::WAVES, 0) != hipSuccess)
     return 0;
   return blocks;
+}
+
+template <class S, bool PRIM, int N, bool PR>
+static int occupancy_inlet(const hype_variant& v) {
+  if (v.inlet) {
+    if constexpr (S::HAS_INLET) return occupancy_variant<S, PRIM, N, PR, true>();
+    return 0;
+  }
+  return occupancy_variant<S, PRIM, N, PR, false>();
 }
 
 template <class S, bool PRIM, int N>
 static int occupancy_layout(const hype_variant& v) {
   if (v.pair) {
-    if constexpr ((S::PAIR_MODES & 2) != 0) return occupancy_variant<S, PRIM, N, true>();
+    if constexpr ((S::PAIR_MODES & 2) != 0) return occupancy_inlet<S, PRIM, N, true>(v);
   } else {
-    if constexpr ((S::PAIR_MODES & 1) != 0) return occupancy_variant<S, PRIM, N, false>();
+    if constexpr ((S::PAIR_MODES & 1) != 0) return occupancy_inlet<S, PRIM, N, false>(v);
   }
   return 0;
 }

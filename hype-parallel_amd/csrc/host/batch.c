@@ -67,7 +67,7 @@ int hype_batch_create(const hype_problem *probs, int n, const hype_batch_opts *o
         b->nv[i] = hype_nvar(probs[i].system);
         b->ncell[i] = (size_t)probs[i].nx * probs[i].ny;
         b->off[i] = total;
-        total += 3 * b->ncell[i] * b->nv[i];
+        total += 3 * b->ncell[i] * b->nv[i] + hype_inlet_doubles(&probs[i]); /* U, A, B, inlet table */
     }
     if ((rc = eng_init(&b->E, n, b->opts.tile_rows, 0)) != HYPE_OK ||
         (rc = hyped_malloc((void **)&b->slab, total * sizeof(double))) != HYPE_OK ||
@@ -78,8 +78,24 @@ int hype_batch_create(const hype_problem *probs, int n, const hype_batch_opts *o
     }
     for (i = 0; i < n; ++i) {
         size_t m = b->ncell[i] * b->nv[i];
-        double *U = b->slab + b->off[i];
-        if ((rc = eng_attach(&b->E, i, &probs[i], U, U + m, U + 2 * m)) != HYPE_OK) {
+        double *U = b->slab + b->off[i], *inlet = NULL;
+        const size_t ni = hype_inlet_doubles(&probs[i]);
+        if (ni > 0) {
+            double *tab = (double *)malloc(ni * sizeof(double));
+            if (!tab) rc = HYPE_E_NOMEM;
+            else {
+                hype_inlet_table(&probs[i], tab);
+                inlet = U + 3 * m;
+                rc = hyped_memcpy_h2d(inlet, tab, ni * sizeof(double), b->E.stream);
+                if (rc == HYPE_OK) rc = hyped_stream_sync(b->E.stream);
+                free(tab);
+            }
+            if (rc != HYPE_OK) {
+                hype_batch_destroy(b);
+                return rc;
+            }
+        }
+        if ((rc = eng_attach(&b->E, i, &probs[i], U, U + m, U + 2 * m, inlet)) != HYPE_OK) {
             hype_batch_destroy(b);
             return rc;
         }

@@ -31,9 +31,13 @@ int hype_check_problem(const hype_problem *p, int *nvar)
     if (p->options & ~(HYPE_OPT_FMA | HYPE_OPT_ONE_WAVE_TILES | HYPE_OPT_SPLIT_TILES)) return HYPE_E_INVALID;
     if ((p->options & HYPE_OPT_ONE_WAVE_TILES) && (p->options & HYPE_OPT_SPLIT_TILES)) return HYPE_E_INVALID;
     for (s = 0; s < 4; ++s) {
-        if (p->bc[s] == HYPE_BC_WALL || p->bc[s] == HYPE_BC_INLET) return HYPE_E_UNSUPPORTED;
+        if (p->bc[s] == HYPE_BC_WALL) return HYPE_E_UNSUPPORTED;
+        /* the reference has an inlet on the left boundary of the Kapila
+         * system only (rhs_function_prim.c:135-144; InletBC is defined in
+         * kapila_5eqn.c alone) */
+        if (p->bc[s] == HYPE_BC_INLET && (s != 0 || p->system != HYPE_KAPILA)) return HYPE_E_UNSUPPORTED;
         if (p->bc[s] != HYPE_BC_PERIODIC && p->bc[s] != HYPE_BC_REFLECTIVE &&
-            p->bc[s] != HYPE_BC_TRANSMISSIVE)
+            p->bc[s] != HYPE_BC_TRANSMISSIVE && p->bc[s] != HYPE_BC_INLET)
             return HYPE_E_INVALID;
     }
     if (nvar) *nvar = nv;

@@ -21,6 +21,11 @@ int hype_strips(int nx);
  * `occ` resident waves of the kernel per SIMD; pair: each tile runs on two
  * waves, one per half (tile-shape model in common.c) */
 int hype_pick_segments(int nx, int ny, int simds, int occ, int extra_waves, int pair);
+/* left inlet (Kapila): doubles of the per-row table (0 without an inlet),
+ * and the table itself, rows j = -3 .. ny+2 of nvar values each, primitive
+ * or conserved as the problem's path stores ghost cells */
+size_t hype_inlet_doubles(const hype_problem *p);
+void hype_inlet_table(const hype_problem *p, double *out);
 /* tile layouts a problem may use (bit 0: one wave per tile, bit 1: split
  * tiles), given those its system's kernel is built for */
 int hype_layouts(const hype_problem *p, int built);

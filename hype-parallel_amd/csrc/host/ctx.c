@@ -29,6 +29,7 @@ struct hype_ctx {
     double *hist_d;
     int hist_cap;
     unsigned char *fb_d;
+    double *inlet_d;        /* left-inlet table (device), or NULL */
     hype_inst inst_h;
     void *graph;            /* one captured step */
     unsigned flags;
@@ -50,6 +51,7 @@ static void ctx_free(hype_ctx *c)
     hyped_free(c->queue_d);
     hyped_free(c->hist_d);
     hyped_free(c->fb_d);
+    hyped_free(c->inlet_d);
     if (c->stream) hyped_stream_destroy(c->stream);
     free(c);
 }
@@ -89,6 +91,7 @@ int hype_ctx_create(const hype_problem *p, hype_ctx **out)
             v.order = p->order;
             v.fma = (p->options & HYPE_OPT_FMA) != 0;
             v.pair = pr;
+            v.inlet = p->bc[0] == HYPE_BC_INLET;
             slots = ncu * hyped_blocks_per_cu(&v) * hyped_block_waves(&v); /* resident waves */
             if (slots < 1) slots = 4 * ncu;
             occ = (slots + 4 * ncu - 1) / (4 * ncu);
@@ -125,6 +128,18 @@ int hype_ctx_create(const hype_problem *p, hype_ctx **out)
     c->inst_h.buf[2] = c->buf[2];
     c->inst_h.buf[3] = NULL;
     c->inst_h.st = c->st;
+    if (hype_inlet_doubles(p) > 0) {
+        const size_t n = hype_inlet_doubles(p);
+        double *tab = (double *)malloc(n * sizeof(double));
+        if (!tab) { rc = HYPE_E_NOMEM; goto fail; }
+        hype_inlet_table(p, tab);
+        rc = hyped_malloc((void **)&c->inlet_d, n * sizeof(double));
+        if (rc == HYPE_OK) rc = hyped_memcpy_h2d(c->inlet_d, tab, n * sizeof(double), c->stream);
+        if (rc == HYPE_OK) rc = hyped_stream_sync(c->stream);
+        free(tab);
+        if (rc != HYPE_OK) goto fail;
+        c->inst_h.inlet = c->inlet_d;
+    }
     if ((rc = hyped_memcpy_h2d(c->inst_d, &c->inst_h, sizeof(hype_inst), c->stream)) != HYPE_OK) goto fail;
     tiles = (hype_tile *)malloc((size_t)c->ntiles * sizeof(hype_tile));
     if (!tiles) { rc = HYPE_E_NOMEM; goto fail; }
@@ -155,6 +170,7 @@ static hype_launch base_launch(hype_ctx *c, int mode, int stage)
     L.v.order = c->P.order;
     L.v.fma = (c->P.options & HYPE_OPT_FMA) != 0;
     L.v.pair = c->pair;
+    L.v.inlet = c->P.bc[0] == HYPE_BC_INLET;
     L.insts = c->inst_d;
     L.tiles = c->tiles_d;
     L.ntiles = c->ntiles;

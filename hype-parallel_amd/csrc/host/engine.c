@@ -63,7 +63,7 @@ void eng_free(hype_engine *E)
 static int same_variant(const hype_variant *a, const hype_variant *b)
 {
     return a->system == b->system && a->prim == b->prim && a->order == b->order && a->fma == b->fma &&
-           a->pair == b->pair;
+           a->pair == b->pair && a->inlet == b->inlet;
 }
 
 static int find_group(hype_engine *E, const hype_variant *v)
@@ -100,7 +100,8 @@ static int find_group(hype_engine *E, const hype_variant *v)
     }
 }
 
-int eng_attach(hype_engine *E, int slot, const hype_problem *p, double *U, double *A, double *B)
+int eng_attach(hype_engine *E, int slot, const hype_problem *p, double *U, double *A, double *B,
+               const double *inlet)
 {
     hype_variant v;
     int g, nv = 0;
@@ -119,6 +120,7 @@ int eng_attach(hype_engine *E, int slot, const hype_problem *p, double *U, doubl
         if (!modes) return HYPE_E_UNSUPPORTED;
         v.pair = (modes & 1) ? 0 : 1;
     }
+    v.inlet = p->bc[0] == HYPE_BC_INLET;
     g = find_group(E, &v);
     if (g < 0) return HYPE_E_DEVICE;
     E->P[slot] = *p;
@@ -132,6 +134,7 @@ int eng_attach(hype_engine *E, int slot, const hype_problem *p, double *U, doubl
     E->inst_h[slot].buf[2] = B;
     E->inst_h[slot].buf[3] = NULL;
     E->inst_h[slot].st = E->st_d + slot;
+    E->inst_h[slot].inlet = inlet;
     return HYPE_OK;
 }
 

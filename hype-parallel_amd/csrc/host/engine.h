@@ -15,7 +15,7 @@
 
 #include "common.h"
 
-#define HYPE_MAX_GROUPS 36  /* system x path x order 1..3 x contraction */
+#define HYPE_MAX_GROUPS 48  /* system x path x order 1..3 x contraction, plus inlet builds */
 
 typedef struct {
     hype_variant v;
@@ -54,7 +54,8 @@ typedef struct {
 int eng_init(hype_engine *E, int cap, int target_rows, unsigned flags);
 void eng_free(hype_engine *E);
 /* place an instance in a slot (its U, A, B device buffers) */
-int eng_attach(hype_engine *E, int slot, const hype_problem *p, double *U, double *A, double *B);
+int eng_attach(hype_engine *E, int slot, const hype_problem *p, double *U, double *A, double *B,
+               const double *inlet);
 int eng_detach(hype_engine *E, int slot);
 /* include (on = 1) or leave out a used slot from the steps; takes effect at
  * the next eng_commit.  A left-out instance keeps its state and time. */

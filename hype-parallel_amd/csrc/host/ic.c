@@ -56,6 +56,59 @@ static void prim2cons(const hype_problem *p, const double *V, double *Q)
     }
 }
 
+/* ---- left inlet of the Kapila system ------------------------------------
+ * InletBCPrim (kapila_5eqn.c:485-513); InletBC (:289-320) is PDEPrim2Cons of
+ * the same state.  The profile depends on the row's height only, so the host
+ * tabulates it once per ghosted row with the C library's sqrt and tanh, the
+ * functions the reference calls. */
+static void inlet_prim(const hype_problem *p, double y, double *V)
+{
+    const double M = 2.0;
+    const double rho_air = 1.0;
+    const double rho_water = 1000.0;
+    const double Pr = 1.013;
+    const double U_avg = M * sqrt(p->gamma2 * Pr / rho_air);
+    const double phi_R = 1.0 - 1.0e-6;
+    const double phi_L = 1.0e-6;
+    const double PHI = 0.5 * (phi_R + phi_L);
+    const double dif_phi = (phi_L - phi_R) / 2.0;
+    const double hh = 30.0 / 500.0;
+    const double epsilon = 3.0 * hh;
+    const double r_0 = 1.0;
+    const double r = fabs(y);
+    V[0] = rho_water;
+    V[1] = rho_air;
+    V[2] = (r < 1.0) ? 2.0 * U_avg * (1.0 - y * y) : 0.0;
+    V[3] = 0.0;
+    V[4] = Pr;
+    V[5] = PHI + dif_phi * tanh(-(r - r_0) / epsilon);
+}
+
+size_t hype_inlet_doubles(const hype_problem *p)
+{
+    if (p->bc[0] != HYPE_BC_INLET) return 0;
+    return (size_t)(p->ny + 6) * (size_t)hype_nvar(p->system);
+}
+
+void hype_inlet_table(const hype_problem *p, double *out)
+{
+    const double h = (p->x_max - p->x_min) / (double)(p->nx); /* hype.c:57 */
+    const int nv = hype_nvar(p->system);
+    int j;
+    for (j = -3; j < p->ny + 3; ++j) {
+        /* cell-centre height of the (ghost) row, rhs_function_prim.c:138 */
+        const double y_loc = p->y_min + ((double)(j) + 0.5) * h;
+        double V[9];
+        double *d = out + (size_t)(j + 3) * nv;
+        int c;
+        inlet_prim(p, y_loc, V);
+        if (p->recons_prim)
+            for (c = 0; c < nv; ++c) d[c] = V[c];
+        else
+            prim2cons(p, V, d);
+    }
+}
+
 /* ---- splitmix64 / xoshiro256** (seeded parameters) -------------------- */
 static uint64_t splitmix64(uint64_t *s)
 {

@@ -49,6 +49,7 @@ typedef struct {
     int state;
     int slot;
     size_t off, bytes;
+    double *inlet_h;        /* left-inlet table while resident (host copy), or NULL */
     double t_submit, t_start, t_done;
     hype_status st;
 } srv_req;
@@ -164,7 +165,9 @@ static size_t state_bytes(const hype_problem *p)
 
 void hype_server_destroy(hype_server *s)
 {
+    int k;
     if (!s) return;
+    for (k = 0; k < s->nreq; ++k) free(s->req[k].inlet_h);
     eng_free(&s->E);
     arena_free_all(&s->A);
     if (s->ev0) hyped_event_destroy(s->ev0);
@@ -247,7 +250,8 @@ int hype_server_submit(hype_server *s, const hype_request *r, int *id)
     if (!s || !r || !r->U0) return HYPE_E_INVALID;
     HYPE_TRY(hype_check_problem(&r->problem, &nv));
     if (r->problem.order < 1) return HYPE_E_UNSUPPORTED;
-    if (round_up(3 * state_bytes(&r->problem)) > s->A.size) return HYPE_E_NOMEM; /* never admissible */
+    if (round_up(3 * state_bytes(&r->problem) + hype_inlet_doubles(&r->problem) * sizeof(double)) > s->A.size)
+        return HYPE_E_NOMEM; /* never admissible */
     if (s->nreq == s->capreq) {
         int cap = s->capreq ? 2 * s->capreq : 256;
         srv_req *nr = (srv_req *)realloc(s->req, (size_t)cap * sizeof(srv_req));
@@ -264,7 +268,8 @@ int hype_server_submit(hype_server *s, const hype_request *r, int *id)
     q->r = *r;
     q->state = HYPE_REQ_QUEUED;
     q->slot = -1;
-    q->bytes = 3 * state_bytes(&r->problem);
+    q->bytes = 3 * state_bytes(&r->problem) + hype_inlet_doubles(&r->problem) * sizeof(double);
+    q->inlet_h = NULL;
     q->t_submit = now_s(s);
     if (id) *id = s->nreq;
     s->queue[s->nqueue++] = s->nreq;
@@ -365,6 +370,8 @@ static int retire(hype_server *s, int slot, double t)
     q->slot = -1;
     HYPE_TRY(eng_detach(&s->E, slot));
     HYPE_TRY(arena_release(&s->A, q->off, q->bytes));
+    free(q->inlet_h);
+    q->inlet_h = NULL;
     s->slot_req[slot] = -1;
     s->active_cells -= (long long)q->r.problem.nx * q->r.problem.ny;
     --s->active;
@@ -442,10 +449,27 @@ int hype_server_poll(hype_server *s, int *active, int *queued)
         if (arena_alloc(&s->A, q->bytes, &q->off) != 0) break;
         U = (double *)(s->A.base + q->off);
         m = (size_t)cells * hype_nvar(q->r.problem.system);
-        if ((rc = hyped_memcpy_h2d(U, q->r.U0, m * sizeof(double), s->E.stream)) != HYPE_OK ||
-            (rc = eng_attach(&s->E, slot, &q->r.problem, U, U + m, U + 2 * m)) != HYPE_OK) {
-            arena_release(&s->A, q->off, q->bytes);
-            goto out;
+        {
+            const size_t ni = hype_inlet_doubles(&q->r.problem);
+            double *inlet = NULL;
+            if (ni > 0) {  /* the table's host copy lives until the request retires */
+                q->inlet_h = (double *)malloc(ni * sizeof(double));
+                if (!q->inlet_h) {
+                    arena_release(&s->A, q->off, q->bytes);
+                    rc = HYPE_E_NOMEM;
+                    goto out;
+                }
+                hype_inlet_table(&q->r.problem, q->inlet_h);
+                inlet = U + 3 * m;
+                rc = hyped_memcpy_h2d(inlet, q->inlet_h, ni * sizeof(double), s->E.stream);
+            }
+            if (rc != HYPE_OK || (rc = hyped_memcpy_h2d(U, q->r.U0, m * sizeof(double), s->E.stream)) != HYPE_OK ||
+                (rc = eng_attach(&s->E, slot, &q->r.problem, U, U + m, U + 2 * m, inlet)) != HYPE_OK) {
+                arena_release(&s->A, q->off, q->bytes);
+                free(q->inlet_h);
+                q->inlet_h = NULL;
+                goto out;
+            }
         }
         q->slot = slot;
         q->state = HYPE_REQ_RUNNING;

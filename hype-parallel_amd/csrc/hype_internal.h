@@ -55,6 +55,8 @@ typedef struct hype_inst {
     double g1, g2, p1, p2;
     unsigned quirks;
     int keybits;           /* face-key bits of the packed error word (28 or 36) */
+    const double *inlet;   /* left inlet: ghost values per row j = -3 .. ny+2
+                              (nvar each, in the path's variables), or NULL */
 } hype_inst;
 
 /* tile: instance, first owned column and row, owned width (<= HYPE_TILE_W)
@@ -84,6 +86,7 @@ typedef struct hype_variant {
     int system, prim, order;
     int fma;               /* 1: the FMA-contracted kernel build (HYPE_OPT_FMA) */
     int pair;              /* 1: each tile split over a two-wave workgroup */
+    int inlet;             /* 1: the build with left-inlet ghost columns */
 } hype_variant;
 
 typedef struct hype_launch {

@@ -43,7 +43,9 @@ extern "C" {
 
 /* ---- boundary kinds, same values as enum bndry_type (hype.h:76) ------- */
 #define HYPE_BC_WALL 0         /* no implementation in the reference: rejected */
-#define HYPE_BC_INLET 1        /* hard-coded jet profile: rejected */
+#define HYPE_BC_INLET 1        /* left boundary of the Kapila system only: the
+                                  reference's fixed jet profile (InletBC /
+                                  InletBCPrim, kapila_5eqn.c:289-320,485-513) */
 #define HYPE_BC_PERIODIC 2
 #define HYPE_BC_REFLECTIVE 3
 #define HYPE_BC_TRANSMISSIVE 4

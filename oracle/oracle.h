@@ -98,6 +98,8 @@ int orc_solve(const orc_problem *P, double *U, double t0, double t_end,
 /* Point functions exported for the pinning tests. */
 void orc_cons2prim(const orc_problem *P, const double *Q, double *V);
 void orc_prim2cons(const orc_problem *P, const double *V, double *Q);
+/* primitive state of the Kapila left inlet at height y (kapila_5eqn.c:485-513) */
+void orc_inlet_prim(const orc_problem *P, double y, double *V);
 double orc_flux(const orc_problem *P, const double *Q, double nx, double ny,
                 double *F, int *bad);
 double orc_flux_prim(const orc_problem *P, const double *V, double nx,

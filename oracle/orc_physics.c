@@ -563,6 +563,35 @@ static int bn_pad_prim(const orc_problem *P, const double *V)
     return PAD;
 }
 
+/* ------------------------------------------------------- Kapila inlet -- */
+
+/* Left inlet of the Kapila system (kapila_5eqn.c:485-513; InletBC at :289-320
+ * is PDEPrim2Cons of the same state): a Mach-2 water jet of parabolic
+ * velocity profile in |y| < 1, air pressure 1.013, and a tanh volume-fraction
+ * profile of width 3*30/500 around |y| = 1.  Independent of x and t. */
+void orc_inlet_prim(const orc_problem *P, double y, double *V)
+{
+    const double M = 2.0;
+    const double rho_air = 1.0;
+    const double rho_water = 1000.0;
+    const double Pr = 1.013;
+    const double U_avg = M * sqrt(P->g2 * Pr / rho_air);
+    const double phi_R = 1.0 - 1.0e-6;
+    const double phi_L = 1.0e-6;
+    const double PHI = 0.5 * (phi_R + phi_L);
+    const double dif_phi = (phi_L - phi_R) / 2.0;
+    const double hh = 30.0 / 500.0;
+    const double epsilon = 3.0 * hh;
+    const double r_0 = 1.0;
+    const double r = fabs(y);
+    V[0] = rho_water;
+    V[1] = rho_air;
+    V[2] = (r < 1.0) ? 2.0 * U_avg * (1.0 - y * y) : 0.0;
+    V[3] = 0.0;
+    V[4] = Pr;
+    V[5] = PHI + dif_phi * tanh(-(r - r_0) / epsilon);
+}
+
 /* ------------------------------------------------------------ dispatch -- */
 
 void orc_cons2prim(const orc_problem *P, const double *Q, double *V)

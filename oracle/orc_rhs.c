@@ -115,9 +115,11 @@ static int check_problem(const orc_problem *P)
     if (P->nx < 3 || P->ny < 3) return ORC_E_INVALID;
     if (!(P->h > 0.0)) return ORC_E_INVALID;
     for (s = 0; s < 4; ++s) {
-        if (P->bc[s] == ORC_BC_WALL || P->bc[s] == ORC_BC_INLET)
-            return ORC_E_UNSUPPORTED; /* wall has no implementation (hype.h:76);
-                                         inlet is a hard-coded jet profile */
+        if (P->bc[s] == ORC_BC_WALL)
+            return ORC_E_UNSUPPORTED; /* no implementation (hype.h:76) */
+        /* the inlet exists on the left boundary of the Kapila system only
+         * (rhs_function_prim.c:135-144; InletBC in kapila_5eqn.c alone) */
+        if (P->bc[s] == ORC_BC_INLET && (s != 0 || P->system != ORC_KAPILA)) return ORC_E_UNSUPPORTED;
         if (P->bc[s] < 0 || P->bc[s] > 4) return ORC_E_INVALID;
     }
     return ORC_OK;
@@ -184,6 +186,15 @@ static void fill_local(orc_ctx *C, const double *src)
             if (i >= nx) { side = 1; }
             if (side < 0) continue;
             int bc = P->bc[side];
+            if (bc == ORC_BC_INLET) { /* rhs_function_prim.c:135-144 / rhs_function.c:135-144 */
+                double V[ORC_MAXVAR];
+                const double y_loc = P->y_min + ((double)(j) + 0.5) * P->h;
+                double *d = LOC(C, j, i);
+                orc_inlet_prim(P, y_loc, V);
+                if (P->prim) for (c = 0; c < nv; ++c) d[c] = V[c];
+                else orc_prim2cons(P, V, d);
+                continue;
+            }
             if (bc == ORC_BC_TRANSMISSIVE) irhs = side == 0 ? 0 : nx - 1;
             else if (bc == ORC_BC_REFLECTIVE) irhs = side == 0 ? -1 - i : 2 * (nx - 1) - i + 1;
             else continue;

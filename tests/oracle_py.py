@@ -15,6 +15,8 @@ ORACLE_DIR = os.path.join(ROOT, "oracle")
 
 EULER, KAPILA, BN = 0, 1, 2
 BC_WALL, BC_INLET, BC_PERIODIC, BC_REFLECTIVE, BC_TRANSMISSIVE = range(5)
+OK, E_INVALID, E_UNSUPPORTED, E_INADMISSIBLE = 0, 1, 2, 5  # oracle.h status codes
+ORC_MAXVAR_PY = 16
 NVAR = {EULER: 4, KAPILA: 6, BN: 9}
 QUIRK_BN_P2_IS_PHI = 0x1
 
@@ -75,6 +77,8 @@ def lib(threaded=False):
     L.orc_cons2prim.restype = None
     L.orc_prim2cons.argtypes = [P(OrcProblem), dp, dp]
     L.orc_prim2cons.restype = None
+    L.orc_inlet_prim.argtypes = [P(OrcProblem), C.c_double, dp]
+    L.orc_inlet_prim.restype = None
     L.orc_flux.argtypes = [P(OrcProblem), dp, C.c_double, C.c_double, dp, P(C.c_int)]
     L.orc_flux.restype = C.c_double
     L.orc_flux_prim.argtypes = [P(OrcProblem), dp, C.c_double, C.c_double, dp, P(C.c_int)]
@@ -101,6 +105,22 @@ def lib(threaded=False):
 
 def dptr(a):
     return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def inlet_prim(P, y):
+    """Primitive state of the Kapila left inlet at height y."""
+    V = np.zeros(ORC_MAXVAR_PY, dtype=np.float64)
+    lib().orc_inlet_prim(C.byref(P), float(y), dptr(V))
+    return V[:NVAR[P.system]].copy()
+
+
+def prim2cons(P, V):
+    """PDEPrim2Cons of one primitive state."""
+    nv = NVAR[P.system]
+    V = np.ascontiguousarray(V, dtype=np.float64)
+    Q = np.zeros(nv, dtype=np.float64)
+    lib().orc_prim2cons(C.byref(P), dptr(V), dptr(Q))
+    return Q
 
 
 def problem(system, nx, ny, *, N=2, prim=True, x_min=0.0, y_min=0.0, h=None, cfl=0.5,

@@ -127,3 +127,34 @@ def rough_state(system, nx, ny, seed=0):
     phi = np.where(salt, 1e-6, 1 - 1e-6)
     return bn_cons(1000.0 * one, 0.2 * one, 0.0 * one, 1e5 * one, one, 0.1 * one, 0.3 * one, 1e5 * one, phi,
                    4.4, 1.4, 6e8, 0.0)
+
+
+# Kapila jet setting of the reference's left inlet: water (gamma 4.4, pi 6000)
+# and air (gamma 1.4), the EOS constants of hype.h:42-45
+INLET_EOS = dict(gamma1=4.4, gamma2=1.4, pi1=6000.0, pi2=0.0)
+
+
+def inlet_state(orc, P, seed=0, amp=0.05):
+    """Kapila state near the left-inlet profile: every column carries the
+    inlet's primitive state at the row's height, smoothly perturbed (density,
+    velocity and pressure within a few per cent), converted with the oracle's
+    PDEPrim2Cons.  P is an oracle problem (x_min, y_min, h, nx, ny)."""
+    rng = np.random.default_rng(seed)
+    nx, ny = P.nx, P.ny
+    x = P.x_min + (np.arange(nx) + 0.5) * P.h
+    y = P.y_min + (np.arange(ny) + 0.5) * P.h
+    k1, k2 = rng.uniform(0.5, 2.0, 2)
+    ph = rng.uniform(0, 2 * np.pi, 3)
+    U = np.empty((ny, nx, 6))
+    for j in range(ny):
+        V0 = orc.inlet_prim(P, y[j])
+        for i in range(nx):
+            s = np.sin(k1 * x[i] + ph[0]) * np.cos(k2 * y[j] + ph[1])
+            V = V0.copy()
+            V[0] *= 1.0 + amp * s
+            V[1] *= 1.0 + amp * np.cos(k1 * x[i] - ph[2])
+            V[2] += amp * 3.0 * s
+            V[3] += amp * 2.0 * np.sin(k2 * y[j] + ph[2])
+            V[4] *= 1.0 + amp * s
+            U[j, i] = orc.prim2cons(P, V)
+    return U
