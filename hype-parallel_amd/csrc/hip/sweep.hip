@@ -21,14 +21,19 @@
 //     fused; the stage output goes straight to HBM, and the wave's largest
 //     wave speed feeds the device-wide CFL maximum (:617-623).
 //
-// With split tiles (PR) the kernel runs two waves per workgroup.  A tile of two or more
-// rows is split at its middle row ym; both waves march up.  The lower wave
-// starts on the ring row below the tile as above and stops at row ym-1.
-// The upper wave starts at ym with no ring row below it.  It sets aside row
-// ym's bottom-face values and the assembly of row ym until both halves are
-// done; then the two waves swap the values either side of face ym-1/2
-// through LDS (one workgroup barrier), each computes that face's flux and
-// assembles its boundary row.  One reconstructed ring row per wave is saved.
+// The build with split tiles (template parameter PR) runs two waves per
+// workgroup.  A tile of two or more rows is split at its middle row ym; both
+// waves march up.  The lower wave starts on the ring row below the tile as
+// above and stops at row ym-1.  The upper wave starts at ym with no ring row
+// below it.  It sets aside row ym's bottom-face values and the assembly of
+// row ym until both halves are done; then the two waves swap the values
+// either side of face ym-1/2 through LDS (one workgroup barrier), each
+// computes that face's flux and assembles its boundary row.  One
+// reconstructed ring row per wave is saved.  Each system declares the
+// layouts it is built for (S::PAIR_MODES); the host picks one per problem.
+//
+// The build with the left inlet (template parameter IN, Kapila) writes the
+// host-tabulated inlet state into the window columns left of the grid.
 //
 // Arithmetic follows the reference's expression trees and is compiled
 // without FMA contraction, so results equal the CPU restatement bit for bit.
@@ -244,7 +249,7 @@ struct Tile {
     const bool owned = lane >= 1 && lane <= W;
     const bool xlane = lane >= 1 && lane <= W + 1;
     const int cw = lane + 2;  // window column of this lane's cell
-    // split tile (paired systems, two or more rows): the lower wave owns
+    // split tile (split-tile build, two or more rows): the lower wave owns
     // rows [y0, ym), the upper wave [ym, y0 + H); a one-row tile is the
     // lower wave's alone
     const bool split = PAIR && H >= 2;
