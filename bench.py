@@ -328,14 +328,17 @@ def run_served(args):
             ctx.set_state(U0)
             ctx.begin(0.0, t_end, cap, H.TS_CLIP | H.TS_GRAPH)
             ctx.advance(cap // 2)  # the requested step count; more if t_end is not reached
-            st = ctx.sync()
+            # a request may end on an inadmissible state (the reference would
+            # abort there); the server reports it, and so must the solo run
+            st = ctx.sync(allow_inadmissible=True)
             while not st.done:
                 ctx.advance(32)
-                st = ctx.sync()
+                st = ctx.sync(allow_inadmissible=True)
             Us = ctx.get_state()
         dur = time.perf_counter() - t0
         busy += dur
-        identical &= bool(np.array_equal(Us, U)) and st.steps == inf.status.steps
+        identical &= (bool(np.array_equal(Us, U, equal_nan=True)) and st.steps == inf.status.steps
+                      and st.err == inf.status.err)
         finish = max(finish, arr) + dur
         solo_tts.append(finish - arr)
     solo_tts = np.array(solo_tts)
@@ -356,6 +359,7 @@ def run_served(args):
                           "p99": round(float(np.percentile(solo_tts, 99)), 4),
                           "makespan": round(float(finish), 3)},
         "identical_to_solo": identical,
+        "inadmissible_requests": int(sum(1 for _, inf in res if inf.status.err)),
     }
     print(json.dumps(out))
 
