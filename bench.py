@@ -160,13 +160,22 @@ def valu_roofline(workload_name, launch_s):
             "unit": "G wave-instructions/s", "frac": round(achieved / peak, 4), "issue_slots_per_launch": slots}
 
 
-def aggregate_roofline(cell_steps_by_variant, seconds):
-    """Algorithmic HBM bytes of all stage launches over the timed region."""
+def aggregate_roofline(cell_steps_by_variant, seconds, workload_name=None):
+    """Algorithmic HBM bytes of all stage launches over the timed region;
+    traffic (HBM bytes per launch, rocprofv3) where the workload has a
+    committed profile."""
     total = sum(n * bytes_per_cell_step(nv, order) for (nv, order), n in cell_steps_by_variant.items())
     achieved = total / seconds / 1e9
     return {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None, "kernel": "stage_kernel (all variants)",
-            "algorithmic_bytes": int(total)}
+            "frac": round(achieved / HBM_PEAK_GBS, 5),
+            "traffic": load_traffic(workload_name) if workload_name else None,
+            "kernel": "stage_kernel (all variants)", "algorithmic_bytes": int(total)}
+
+
+def per_launch(roofline, launches):
+    """Add the algorithmic bytes per stage launch (the unit of `traffic`)."""
+    roofline["algorithmic_bytes_per_launch"] = int(roofline["algorithmic_bytes"] / launches)
+    return roofline
 
 
 def kapila_bubble(n, mach, R, cx, cy):
@@ -182,6 +191,9 @@ def kapila_bubble(n, mach, R, cx, cy):
 
 def progress(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+BATCHED_WORKLOAD = "batched_kapila_64x_loguniform_128_1024"
 
 
 def run_batched(args):
@@ -233,7 +245,8 @@ def run_batched(args):
         "config": {"workload": f"batched_kapila_{count}x_loguniform_128_1024", "instances": count,
                    "cells": cells, "order": 2, "seed": args.seed, "tile_rows": rows, "tiles": ntiles,
                    "variants": groups},
-        "roofline": aggregate_roofline({(6, 2): cells * steps}, sec),
+        "roofline": per_launch(aggregate_roofline({(6, 2): cells * steps}, sec, BATCHED_WORKLOAD), 4 * steps),
+        "valu_roofline": valu_roofline(BATCHED_WORKLOAD, sec / (4 * steps)),
         "one_at_a_time": {"value": round(cells * steps / (solo_ms / 1e3) / 1e6, 3), "unit": "Mcell-updates/s",
                           "ms": round(solo_ms, 3)},
         "speedup_vs_one_at_a_time": round(solo_ms / ms, 3),
