@@ -140,6 +140,19 @@ __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// global-memory view of a pointer.  The state, output and inlet pointers are
+// read from the instance table, so the compiler cannot infer their address
+// space and would use flat loads and stores.  Those count on the LDS wait
+// counter too, so every LDS read or lane shuffle of a row iteration would
+// wait for the next row's prefetch; global instructions count on the vector
+// memory counter alone.
+template <class T>
+using gmem_t = __attribute__((address_space(1))) T;
+template <class T>
+__device__ __forceinline__ gmem_t<T>* gmem(T* p) {
+  return (gmem_t<T>*)p;
+}
+
 constexpr int NSLOT = 5;
 constexpr int COLS = HYPE_WAVE + 4;
 
@@ -239,7 +252,7 @@ struct Tile {
         st->err[(cur + 1) & 1] = ~0ull;  // the next step records there
       }
     }
-    const double* Uin = (mode == HYPE_MODE_STAGE) ? I->buf[sd.in] : (L.rhs_in ? L.rhs_in : I->buf[0]);
+    const auto Uin = gmem<const double>((mode == HYPE_MODE_STAGE) ? I->buf[sd.in] : (L.rhs_in ? L.rhs_in : I->buf[0]));
     const double cdt = dt / sd.bden;
 
     // ---- geometry ------------------------------------------------------
@@ -286,7 +299,7 @@ struct Tile {
       bool fx = false, fy = false;
       const int sc = bc_map(cc, nx, bcL, bcR, fx);  // an inlet column maps as periodic; put_inlet replaces it
       const int sr = bc_map(rr, ny, bcB, bcT, fy);
-      const double* p = Uin + ((size_t)sr * nx + sc) * NV;
+      const auto p = Uin + ((size_t)sr * nx + sc) * NV;
 #pragma unroll
       for (int c = 0; c < NV; ++c) q[c] = p[c];
       return (fx ? 1 : 0) | (fy ? 2 : 0);
@@ -322,7 +335,7 @@ struct Tile {
           bool fy = false;
           const int sr = bc_map(rr, ny, bcB, bcT, fy);
           const bool own = (rr < 0 && bcB == HYPE_BC_PERIODIC) || (rr >= ny && bcT == HYPE_BC_PERIODIC);
-          const double* p = I->inlet + (size_t)((own ? rr : sr) + 3) * NV;
+          const auto p = gmem(I->inlet) + (size_t)((own ? rr : sr) + 3) * NV;
           double v[NV];
 #pragma unroll
           for (int c = 0; c < NV; ++c) v[c] = p[c];
@@ -421,11 +434,11 @@ struct Tile {
         rhs[c] = ((S::HAS_NCP && S::DNZ[c]) ? nc[c] : 0.0) + t;
       }
       if (mode == HYPE_MODE_RHS) {
-        double* out = L.rhs_out ? L.rhs_out : I->buf[1];
+        const auto out = gmem(L.rhs_out ? L.rhs_out : I->buf[1]);
 #pragma unroll
         for (int c = 0; c < NV; ++c) out[idx + c] = rhs[c];
       } else {
-        double* out = I->buf[sd.out];
+        const auto out = gmem(I->buf[sd.out]);
         if (sd.op == 0) {
 #pragma unroll
           for (int c = 0; c < NV; ++c) out[idx + c] = Uin[idx + c] + cdt * rhs[c];
@@ -433,7 +446,7 @@ struct Tile {
           // RK104 after its fifth stage: w0 += dt/6 F (VecAXPY); w1 =
           // 1/25 U + 9/25 w0 (VecAXPBYPCZ, c = 0) into U; w0 = 15 w1 - 5 w0
           // (VecAXPBY) into A
-          double* A = I->buf[1];
+          const auto A = gmem(I->buf[1]);
 #pragma unroll
           for (int c = 0; c < NV; ++c) {
             const double w0 = Uin[idx + c] + cdt * rhs[c];
@@ -442,8 +455,8 @@ struct Tile {
             A[idx + c] = 15.0 * w1 + -5.0 * w0;
           }
         } else {
-          const double* X = I->buf[sd.xb];
-          const double* Z = I->buf[sd.zb];
+          const auto X = gmem<const double>(I->buf[sd.xb]);
+          const auto Z = gmem<const double>(I->buf[sd.zb]);
 #pragma unroll
           for (int c = 0; c < NV; ++c) out[idx + c] = sd.a * X[idx + c] + cdt * rhs[c] + sd.g * Z[idx + c];
         }
@@ -526,7 +539,7 @@ struct Tile {
           for (int p = 0; p < NNODE; ++p) an[p] = -1.0;  // new values: the flux computes a_sq itself
         }
       }
-      if (L.fallback && ownedRow && owned) L.fallback[(size_t)r * nx + col] = (unsigned char)(!pad);
+      if (L.fallback && ownedRow && owned) gmem(L.fallback)[(size_t)r * nx + col] = (unsigned char)(!pad);
 
       // ---- slide the window: row r-2 is no longer read, row r+3 takes its
       // slot (the wave's own reads above precede this store in program order)
