@@ -4,7 +4,7 @@ quadrants 256^2 for 200 steps, Baer-Nunziato water-air 2048x1024 for 300
 steps, SSPRK(4,3) with the reference's dt lag), as sha256 of the FP64 state
 bytes with the final time and step count.  tests/test_gpu_fullsize.py holds
 the device to them bit for bit.  The oracle needs about 7 minutes (Kapila)
-and 30 minutes (Baer-Nunziato) on 8 cores.
+and 26 minutes (Baer-Nunziato) on 8 cores.
 
     python tests/golden/make_fullsize_hashes.py [workload ...]
 """
