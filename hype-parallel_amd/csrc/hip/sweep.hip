@@ -13,7 +13,7 @@
 //   * every lane reconstructs its cell (Reconstruct, admissibility check and
 //     minmod fallback, rhs_function_prim.c:321-456) into the values at the
 //     face quadrature points;
-//   * an x-face pairs a lane with its left neighbour through a lane shuffle
+//   * an x-face pairs a lane with its left neighbour through a DPP lane shift
 //     and a y-face pairs a row with the previous row kept in registers, so
 //     each Rusanov flux + non-conservative jump (riemann.c:112-140) is
 //     computed once per face of the tile with no face scratch in memory;
@@ -151,6 +151,22 @@ using gmem_t = __attribute__((address_space(1))) T;
 template <class T>
 __device__ __forceinline__ gmem_t<T>* gmem(T* p) {
   return (gmem_t<T>*)p;
+}
+
+// whole-wave lane shifts by one lane, as DPP moves (wave_shr:1 = 0x138,
+// wave_shl:1 = 0x130): lane i takes lane i-1 (from_left) or lane i+1
+// (from_right), and the end lane keeps its own value, as __shfl_up and
+// __shfl_down do.  Two VALU moves per double, against two LDS permutes and
+// a wait on the LDS counter.
+__device__ __forceinline__ double lane_from_left(double v) {
+  const int lo = __double2loint(v), hi = __double2hiint(v);
+  return __hiloint2double(__builtin_amdgcn_update_dpp(hi, hi, 0x138, 0xf, 0xf, false),
+                          __builtin_amdgcn_update_dpp(lo, lo, 0x138, 0xf, 0xf, false));
+}
+__device__ __forceinline__ double lane_from_right(double v) {
+  const int lo = __double2loint(v), hi = __double2hiint(v);
+  return __hiloint2double(__builtin_amdgcn_update_dpp(hi, hi, 0x130, 0xf, 0xf, false),
+                          __builtin_amdgcn_update_dpp(lo, lo, 0x130, 0xf, 0xf, false));
 }
 
 constexpr int NSLOT = 5;
@@ -583,12 +599,12 @@ struct Tile {
           double vl[NV], vr[NV], Fq[NV], Dq[NV];
 #pragma unroll
           for (int c = 0; c < NV; ++c) {
-            vl[c] = __shfl_up(fv[Qd::R(q)][c], 1, HYPE_WAVE);
+            vl[c] = lane_from_left(fv[Qd::R(q)][c]);
             vr[c] = fv[Qd::L(q)][c];
           }
           double al = -1.0, ar = -1.0;
           if constexpr (ACACHE) {
-            al = __shfl_up(an[Qd::R(q)], 1, HYPE_WAVE);
+            al = lane_from_left(an[Qd::R(q)]);
             ar = an[Qd::L(q)];
           }
           if (ownedRow) {
@@ -611,8 +627,8 @@ struct Tile {
         }
 #pragma unroll
         for (int c = 0; c < NV; ++c) {
-          const double Fr = __shfl_down(Fx[c], 1, HYPE_WAVE);
-          const double Dr = S::DNZ[c] ? __shfl_down(Dx[c], 1, HYPE_WAVE) : 0.0;
+          const double Fr = lane_from_right(Fx[c]);
+          const double Dr = S::DNZ[c] ? lane_from_right(Dx[c]) : 0.0;
           xpart[c] = -r1_h * (Fr - Fx[c] + 0.5 * (Dr + Dx[c]));
         }
       }
