@@ -45,6 +45,22 @@
 #include "physics.hpp"
 #include "recon.hpp"
 
+// Systems this translation unit builds stage kernels for.  The exact build
+// is split in two so that each part gets the scheduler options measured best
+// for it: this file builds Euler and Kapila, sweep_bn.hip (this source with
+// HYPE_SWEEP_BN) builds Baer-Nunziato.  The contracted build (sweep_fc.hip)
+// builds all three.
+#if defined(HYPE_CONTRACTED_BUILD)
+#define HYPE_BUILD_EK 1
+#define HYPE_BUILD_BN 1
+#elif defined(HYPE_SWEEP_BN)
+#define HYPE_BUILD_EK 0
+#define HYPE_BUILD_BN 1
+#else
+#define HYPE_BUILD_EK 1
+#define HYPE_BUILD_BN 0
+#endif
+
 namespace HYPE_KNS {
 
 // --------------------------------------------------------------- schemes --
@@ -877,6 +893,16 @@ static int occupancy_sys(const hype_variant& v) {
   }
 }
 
+#if !HYPE_BUILD_EK
+// the Baer-Nunziato part of the exact build (sweep_bn.hip)
+hipError_t launch_stage_bn(const hype_launch& L, hipStream_t s) { return launch_sys<BN>(L, s); }
+int blocks_per_cu_bn(const hype_variant& v) { return occupancy_sys<BN>(v); }
+#else
+#if !HYPE_BUILD_BN
+hipError_t launch_stage_bn(const hype_launch& L, hipStream_t s);
+int blocks_per_cu_bn(const hype_variant& v);
+#endif
+
 // tile layouts a system's stage kernel is built for (bit 0: one wave per
 // tile, bit 1: split tiles)
 int pair_modes(int system) {
@@ -891,7 +917,11 @@ int blocks_per_cu(const hype_variant& v) {
   switch (v.system) {
     case HYPE_EULER: return occupancy_sys<Euler>(v);
     case HYPE_KAPILA: return occupancy_sys<Kapila>(v);
+#if HYPE_BUILD_BN
     default: return occupancy_sys<BN>(v);
+#else
+    default: return blocks_per_cu_bn(v);
+#endif
   }
 }
 
@@ -900,7 +930,11 @@ hipError_t launch_stage(const hype_launch& L, hipStream_t s) {
   switch (L.v.system) {
     case HYPE_EULER: return launch_sys<Euler>(L, s);
     case HYPE_KAPILA: return launch_sys<Kapila>(L, s);
+#if HYPE_BUILD_BN
     default: return launch_sys<BN>(L, s);
+#else
+    default: return launch_stage_bn(L, s);
+#endif
   }
 }
 
@@ -929,10 +963,11 @@ hipError_t launch_cons2prim(int system, const double* U, double* W, long n, Eos 
     hipLaunchKernelGGL(cons2prim_kernel<BN>, dim3(grid), dim3(bs), 0, s, U, W, n, e);
   return hipGetLastError();
 }
+#endif  // HYPE_BUILD_EK
 
 }  // namespace HYPE_KNS
 
-#ifndef HYPE_CONTRACTED_BUILD
+#if !defined(HYPE_CONTRACTED_BUILD) && HYPE_BUILD_EK
 // The stage table as the host sees it (tests replay the buffer algebra on
 // the CPU against the oracle's TSSSP steps).
 extern "C" int hyped_stage_table(int order, int s, int* ibuf, double* coef) {
