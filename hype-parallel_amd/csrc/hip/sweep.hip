@@ -45,15 +45,12 @@
 #include "physics.hpp"
 #include "recon.hpp"
 
-// Systems this translation unit builds stage kernels for.  The exact build
-// is split in two so that each part gets the scheduler options measured best
-// for it: this file builds Euler and Kapila, sweep_bn.hip (this source with
-// HYPE_SWEEP_BN) builds Baer-Nunziato.  The contracted build (sweep_fc.hip)
-// builds all three.
-#if defined(HYPE_CONTRACTED_BUILD)
-#define HYPE_BUILD_EK 1
-#define HYPE_BUILD_BN 1
-#elif defined(HYPE_SWEEP_BN)
+// Systems this translation unit builds stage kernels for.  Each build (exact
+// and contracted) is split in two so that each part gets the scheduler
+// options measured best for it: this file builds Euler and Kapila,
+// sweep_bn.hip (this source with HYPE_SWEEP_BN) builds Baer-Nunziato, and
+// sweep_fc.hip / sweep_fc_bn.hip do the same for the contracted build.
+#if defined(HYPE_SWEEP_BN)
 #define HYPE_BUILD_EK 0
 #define HYPE_BUILD_BN 1
 #else
