@@ -15,7 +15,10 @@ host cores, bounded sample).
 import argparse
 import json
 import os
+import shutil
+import subprocess
 import sys
+import tempfile
 import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
@@ -99,8 +102,56 @@ def bytes_per_cell_step(nv, order):
     return 5 * nv * 8
 
 
+def mpi_exec():
+    """mpiexec of the MPI installation the oracle's MPI program was built with."""
+    for cand in (shutil.which("mpiexec"), "/opt/conda/bin/mpiexec"):
+        if cand and os.path.exists(cand):
+            return cand
+    return None
+
+
+def cpu_baseline_mpi(p, U0, ranks, budget_s):
+    """The FP64 C oracle with MPI ranks (oracle/orc_mpi_baseline.c: row
+    blocks, width-3 halo exchange per RHS call, Allreduce(MIN) of dt), as the
+    reference runs its PETSc path.  Returns None when it cannot run here."""
+    exe = os.path.join(ROOT, "oracle", "lib", "orc_mpi_baseline")
+    mpiexec = mpi_exec()
+    if not os.path.exists(exe) or mpiexec is None or ranks < 2 or p.ny < 3 * ranks:
+        return None
+    if H.BC_PERIODIC in (p.bc[2], p.bc[3]):
+        return None
+    h = (p.x_max - p.x_min) / p.nx
+    fd, path = tempfile.mkstemp(suffix=".bin")
+    try:
+        with os.fdopen(fd, "wb") as f:
+            np.ascontiguousarray(U0, dtype=np.float64).tofile(f)
+        args = [mpiexec, "-n", str(ranks), exe, path, str(p.system), str(p.order), str(p.recons_prim),
+                str(p.nx), str(p.ny), repr(p.x_min), repr(p.y_min), repr(h), repr(p.cfl)]
+        args += [str(p.bc[k]) for k in range(4)]
+        args += [repr(p.gamma1), repr(p.gamma2), repr(p.pi1), repr(p.pi2), str(p.quirks), repr(budget_s)]
+        r = subprocess.run(args, capture_output=True, text=True, timeout=240)
+        words = r.stdout.split()
+        if r.returncode != 0 or len(words) < 6 or words[0] != "ranks":
+            return None
+        nr, k, sec = int(words[1]), int(words[3]), float(words[5])
+    except (OSError, subprocess.SubprocessError, ValueError):
+        return None
+    finally:
+        os.unlink(path)
+    rate = p.nx * p.ny * k / max(sec, 1e-9) / 1e6
+    return {"value": round(rate, 4), "unit": "Mcell-updates/s", "cores": nr, "kind": "port",
+            "sample": f"{k} SSP-RK steps of the same {p.nx}x{p.ny} workload after its priming call and "
+                      f"first step (C oracle, {nr} MPI ranks: row blocks with a width-3 halo exchange per "
+                      f"RHS call and an Allreduce(MIN) of dt)"}
+
+
 def cpu_baseline(p, U0, nv, budget_s=12.0):
-    """The FP64 C oracle (OpenMP build) on the same workload, bounded sample."""
+    """The FP64 C oracle on the same workload, bounded sample: with MPI ranks
+    on the host cores where MPI is installed, else the OpenMP build."""
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    res = cpu_baseline_mpi(p, U0, threads, budget_s)
+    if res is not None:
+        return res
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_py as orc
 
@@ -115,7 +166,6 @@ def cpu_baseline(p, U0, nv, budget_s=12.0):
     orc.solve(op, U0, k + 1, threaded=True)
     tk = time.perf_counter() - t0
     rate = p.nx * p.ny * k / max(tk - t1, 1e-9) / 1e6
-    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     return {"value": round(rate, 4), "unit": "Mcell-updates/s", "cores": threads, "kind": "port",
             "sample": f"{k} SSP-RK steps of the same {p.nx}x{p.ny} workload from its initial state "
                       f"(C oracle, OpenMP, {threads} threads; priming and allocation cancelled by "
