@@ -22,6 +22,7 @@ struct hype_batch {
     size_t *ncell;
     int *nv;
     hype_batch_opts opts;
+    int began;        /* hype_batch_begin has run: the time states are set */
     void *ev0, *ev1;
     char msg[256];
     double err_x, err_y;
@@ -139,19 +140,20 @@ int hype_batch_begin(hype_batch *b, const double *t0, const double *t_end, const
     for (i = 0; i < b->n; ++i) slots[i] = i;
     rc = eng_start(&b->E, slots, b->n, t0, t_end, max_steps, (flags & HYPE_TS_CLIP) ? 1 : 0, NULL, NULL);
     free(slots);
+    if (rc == HYPE_OK) b->began = 1;
     return rc;
 }
 
 int hype_batch_advance(hype_batch *b, int nsteps)
 {
-    if (!b || nsteps < 0) return HYPE_E_INVALID;
+    if (!b || nsteps < 0 || !b->began) return HYPE_E_INVALID;
     return eng_step(&b->E, nsteps);
 }
 
 int hype_batch_advance_timed(hype_batch *b, int nsteps, double *ms)
 {
     float f = 0.0f;
-    if (!b || nsteps < 0 || !ms) return HYPE_E_INVALID;
+    if (!b || nsteps < 0 || !ms || !b->began) return HYPE_E_INVALID;
     HYPE_TRY(hyped_event_record(b->ev0, b->E.stream));
     HYPE_TRY(eng_step(&b->E, nsteps));
     HYPE_TRY(hyped_event_record(b->ev1, b->E.stream));

@@ -324,6 +324,8 @@ int hype_begin(hype_ctx *c, double t0, double t_end, int max_steps, unsigned fla
     d.hist_len = c->hist_cap;
     d.dt_hist = c->hist_d;
     HYPE_TRY(hyped_memcpy_h2d(c->st, &d, sizeof(d), c->stream));
+    /* d lives on this stack frame: the copy completes before it goes */
+    HYPE_TRY(hyped_stream_sync(c->stream));
     /* priming call (hype.c:343): the wave-speed maximum of U^0 */
     L = base_launch(c, HYPE_MODE_PRIME, 0);
     HYPE_TRY(hyped_launch(&L, c->stream));
@@ -406,33 +408,43 @@ int hype_dt_history(hype_ctx *c, double *dts, int n)
  * near-equal segments) and the persistent grid size */
 int hype_ctx_set_tiling(hype_ctx *c, int hrows, int grid)
 {
-    hype_tile *tiles;
-    int rc, nsy;
+    hype_tile *tiles, *tiles_d = NULL;
+    hype_inst inst;
+    int rc, nsy, ntiles;
     if (!c || hrows < 1 || grid < 0) return HYPE_E_INVALID;
+    nsy = (c->P.ny + hrows - 1) / hrows;
+    ntiles = hype_strips(c->P.nx) * nsy;
+    /* the new table is built and uploaded first; the context switches to it
+     * only when every step succeeded, and keeps the old one otherwise */
+    tiles = (hype_tile *)malloc((size_t)ntiles * sizeof(hype_tile));
+    if (!tiles) return HYPE_E_NOMEM;
+    if (hype_make_tiles(0, c->P.nx, c->P.ny, hype_strips(c->P.nx), nsy, tiles, ntiles) != ntiles) {
+        free(tiles);
+        return HYPE_E_INVALID;
+    }
+    rc = hype_xcd_order(tiles, ntiles, HYPE_NXCD);
+    if (rc == HYPE_OK) rc = hyped_malloc((void **)&tiles_d, (size_t)ntiles * sizeof(hype_tile));
+    if (rc == HYPE_OK) rc = hyped_memcpy_h2d(tiles_d, tiles, (size_t)ntiles * sizeof(hype_tile), c->stream);
+    inst = c->inst_h;
+    inst.nsy = nsy;
+    if (rc == HYPE_OK) rc = hyped_memcpy_h2d(c->inst_d, &inst, sizeof(hype_inst), c->stream);
+    if (rc == HYPE_OK) rc = hyped_stream_sync(c->stream);
+    free(tiles);
+    if (rc != HYPE_OK) {
+        hyped_free(tiles_d);
+        return rc;
+    }
     if (c->graph) {
         hyped_graph_destroy(c->graph);
         c->graph = NULL;
     }
-    nsy = (c->P.ny + hrows - 1) / hrows;
+    hyped_free(c->tiles_d);
+    c->tiles_d = tiles_d;
+    c->inst_h = inst;
     c->nsy = nsy;
     c->grid = grid;
-    c->ntiles = hype_strips(c->P.nx) * nsy;
-    hyped_free(c->tiles_d);
-    c->tiles_d = NULL;
-    HYPE_TRY(hyped_malloc((void **)&c->tiles_d, (size_t)c->ntiles * sizeof(hype_tile)));
-    tiles = (hype_tile *)malloc((size_t)c->ntiles * sizeof(hype_tile));
-    if (!tiles) return HYPE_E_NOMEM;
-    if (hype_make_tiles(0, c->P.nx, c->P.ny, hype_strips(c->P.nx), nsy, tiles, c->ntiles) != c->ntiles) {
-        free(tiles);
-        return HYPE_E_INVALID;
-    }
-    c->inst_h.nsy = nsy;
-    rc = hype_xcd_order(tiles, c->ntiles, HYPE_NXCD);
-    if (rc == HYPE_OK) rc = hyped_memcpy_h2d(c->tiles_d, tiles, (size_t)c->ntiles * sizeof(hype_tile), c->stream);
-    if (rc == HYPE_OK) rc = hyped_memcpy_h2d(c->inst_d, &c->inst_h, sizeof(hype_inst), c->stream);
-    if (rc == HYPE_OK) rc = hyped_stream_sync(c->stream);
-    free(tiles);
-    return rc;
+    c->ntiles = ntiles;
+    return HYPE_OK;
 }
 
 int hype_ctx_tiling(hype_ctx *c, int *hrows, int *ntiles, int *grid)

@@ -28,7 +28,10 @@ int eng_init(hype_engine *E, int cap, int target_rows, unsigned flags)
     HYPE_TRY(hyped_malloc((void **)&E->st_d, (size_t)cap * sizeof(hype_dstate)));
     HYPE_TRY(hyped_stream_create(&E->stream));
     HYPE_TRY(hyped_event_create_sync(&E->ev_fork));
-    return HYPE_OK;
+    /* time states start zeroed: max_steps = 0 and no history pointer, so a
+     * stage launch over a slot that was never started returns at once */
+    HYPE_TRY(hyped_fill_u64(E->st_d, 0ull, (size_t)cap * sizeof(hype_dstate) / 8, E->stream));
+    return hyped_stream_sync(E->stream);
 }
 
 void eng_free(hype_engine *E)
@@ -78,6 +81,7 @@ static int find_group(hype_engine *E, const hype_variant *v)
         G->v = *v;
         G->nstages = hyped_num_stages(v->order);
         G->tile_cap = GROUP_TILE_CAP0;
+        G->prime_cap = GROUP_TILE_CAP0;
         if (hyped_malloc((void **)&G->tiles_d, (size_t)G->tile_cap * sizeof(hype_tile)) != HYPE_OK ||
             hyped_malloc((void **)&G->prime_d, (size_t)G->tile_cap * sizeof(hype_tile)) != HYPE_OK ||
             hyped_malloc((void **)&G->ntiles_d, 2 * sizeof(int)) != HYPE_OK ||
@@ -175,25 +179,42 @@ static int count_tiles(const hype_engine *E, int g, int only_fresh)
 
 int eng_total_tiles(hype_engine *E) { return count_tiles(E, -1, 0); }
 
-/* grow the tile tables of a group to hold n tiles; the stepping table is a
- * kernel argument of the captured graph, so a new one invalidates it */
-static int ensure_cap(hype_engine *E, eng_group *G, int n)
+/* grow the stepping table of a group to hold n tiles; it is a kernel
+ * argument of the captured graph, so a new one invalidates the graph.  The
+ * caller rewrites the whole table afterwards (eng_commit). */
+static int ensure_step_cap(hype_engine *E, eng_group *G, int n)
 {
     int cap = G->tile_cap;
     if (n <= cap) return HYPE_OK;
     while (cap < n) cap *= 2;
     HYPE_TRY(hyped_stream_sync(E->stream));
     hyped_free(G->tiles_d);
-    hyped_free(G->prime_d);
-    G->tiles_d = G->prime_d = NULL;
+    G->tiles_d = NULL;
     G->tile_cap = 0;
+    G->ntiles = 0;
+    HYPE_TRY(hyped_fill_u64(G->ntiles_d, 0ull, 1, E->stream));
     HYPE_TRY(hyped_malloc((void **)&G->tiles_d, (size_t)cap * sizeof(hype_tile)));
-    HYPE_TRY(hyped_malloc((void **)&G->prime_d, (size_t)cap * sizeof(hype_tile)));
     G->tile_cap = cap;
     if (E->graph) {
         hyped_graph_destroy(E->graph);
         E->graph = NULL;
     }
+    return HYPE_OK;
+}
+
+/* grow the priming table of a group; the priming launch is not captured,
+ * and the stepping table and its length are left as they are */
+static int ensure_prime_cap(hype_engine *E, eng_group *G, int n)
+{
+    int cap = G->prime_cap;
+    if (n <= cap) return HYPE_OK;
+    while (cap < n) cap *= 2;
+    HYPE_TRY(hyped_stream_sync(E->stream));
+    hyped_free(G->prime_d);
+    G->prime_d = NULL;
+    G->prime_cap = 0;
+    HYPE_TRY(hyped_malloc((void **)&G->prime_d, (size_t)cap * sizeof(hype_tile)));
+    G->prime_cap = cap;
     return HYPE_OK;
 }
 
@@ -276,7 +297,7 @@ int eng_commit(hype_engine *E)
     for (g = 0; g < E->ng && rc == HYPE_OK; ++g) {
         eng_group *G = &E->g[g];
         int n;
-        if ((rc = ensure_cap(E, G, count_tiles(E, g, 0))) != HYPE_OK) break;
+        if ((rc = ensure_step_cap(E, G, count_tiles(E, g, 0))) != HYPE_OK) break;
         n = build_tiles(E, g, 0, buf, G->tile_cap);
         if (n < 0) {
             rc = HYPE_E_INVALID; /* a grid the tiler cannot split */
@@ -319,30 +340,37 @@ int eng_start(hype_engine *E, const int *slots, int n, const double *t0, const d
               const int *max_steps, int clip, double **hist, const int *hist_len)
 {
     hype_tile *buf;
+    hype_dstate *d;
     int k, g, m, rc = HYPE_OK;
-    for (k = 0; k < n; ++k) {
-        hype_dstate d;
-        int s = slots[k];
-        if (s < 0 || s >= E->cap || !E->used[s]) return HYPE_E_INVALID;
-        memset(&d, 0, sizeof(d));
-        d.t = t0 ? t0[k] : 0.0;
-        d.t_end = t_end[k];
-        d.err[0] = d.err[1] = d.err_rhs = ~0ull;
-        d.max_steps = max_steps[k];
-        d.clip = clip;
-        d.dt_hist = hist ? hist[k] : NULL;
-        d.hist_len = (hist && hist_len) ? hist_len[k] : 0;
-        HYPE_TRY(hyped_memcpy_h2d(E->st_d + s, &d, sizeof(d), E->stream));
+    for (k = 0; k < n; ++k)
+        if (slots[k] < 0 || slots[k] >= E->cap || !E->used[slots[k]]) return HYPE_E_INVALID;
+    /* the new time states: one host array, copied slot by slot and waited
+     * for before the array is freed */
+    d = (hype_dstate *)calloc((size_t)(n > 0 ? n : 1), sizeof(hype_dstate));
+    if (!d) return HYPE_E_NOMEM;
+    for (k = 0; k < n && rc == HYPE_OK; ++k) {
+        const int s = slots[k];
+        d[k].t = t0 ? t0[k] : 0.0;
+        d[k].t_end = t_end[k];
+        d[k].err[0] = d[k].err[1] = d[k].err_rhs = ~0ull;
+        d[k].max_steps = max_steps[k];
+        d[k].clip = clip;
+        d[k].dt_hist = hist ? hist[k] : NULL;
+        d[k].hist_len = (hist && hist_len) ? hist_len[k] : 0;
+        rc = hyped_memcpy_h2d(E->st_d + s, &d[k], sizeof(hype_dstate), E->stream);
         E->fresh[s] = 2; /* marked for priming */
     }
+    if (rc == HYPE_OK) rc = hyped_stream_sync(E->stream);
+    free(d);
+    if (rc != HYPE_OK) return rc;
     for (k = 0; k < E->cap; ++k) E->fresh[k] = (E->fresh[k] == 2);
     m = count_tiles(E, -1, 1);
     buf = (hype_tile *)malloc((size_t)(m > 0 ? m : 1) * sizeof(hype_tile));
     if (!buf) return HYPE_E_NOMEM;
     for (g = 0; g < E->ng && rc == HYPE_OK; ++g) {
         eng_group *G = &E->g[g];
-        if ((rc = ensure_cap(E, G, count_tiles(E, g, 1))) != HYPE_OK) break;
-        m = build_tiles(E, g, 1, buf, G->tile_cap);
+        if ((rc = ensure_prime_cap(E, G, count_tiles(E, g, 1))) != HYPE_OK) break;
+        m = build_tiles(E, g, 1, buf, G->prime_cap);
         if (m < 0) {
             rc = HYPE_E_UNSUPPORTED;
             break;
@@ -357,6 +385,8 @@ int eng_start(hype_engine *E, const int *slots, int n, const double *t0, const d
         }
     }
     for (k = 0; k < E->cap; ++k) E->fresh[k] = 0;
+    /* the tile buffer is pageable host memory: wait for its copies */
+    if (rc == HYPE_OK) rc = hyped_stream_sync(E->stream);
     free(buf);
     return rc;
 }

@@ -25,7 +25,8 @@ typedef struct {
     hype_tile *prime_d;     /* priming table of newly started instances */
     int *nprime_d;
     unsigned *queue_d;      /* {next, finished} for stepping, then priming */
-    int tile_cap;
+    int tile_cap;           /* capacity of the stepping table */
+    int prime_cap;          /* capacity of the priming table */
     int ntiles, nprime;     /* host mirrors */
     int grid;               /* persistent workgroups */
     int bwaves;             /* waves per workgroup (2: tile halves on two waves) */

@@ -179,3 +179,21 @@ def test_batch_with_forced_tile_layouts(hip, tile_rows):
                      g2=eos["gamma2"], p1=eos["pi1"], p2=eos["pi2"])
     ref = orc.solve(op, U0s[0], steps[0])
     assert np.array_equal(out[0], ref["U"])
+
+
+def test_advance_before_begin_is_refused(hip):
+    """The time states are written by begin; stepping before it is an error
+    (and the zeroed states make any stage launch a no-op)."""
+    p = problem(hip, 1, True, 2, 48, 30, (T, T, R, R))
+    with hip.Batch([p, p]) as b:
+        with pytest.raises(hip.HypeError) as e:
+            b.advance(2)
+        assert e.value.code == hip.E_INVALID
+        U0 = states.state(1, 48, 30, seed=3)
+        b.set_state(0, U0)
+        b.set_state(1, U0)
+        b.begin(max_steps=np.array([3, 3], dtype=np.int32))
+        b.advance(4)
+        st = b.sync()
+        assert st[0].steps == st[1].steps == 3
+        assert np.array_equal(b.get_state(0), b.get_state(1))

@@ -124,3 +124,30 @@ def test_failing_request_is_reported_and_isolated(hip):
     assert infb.status.steps == rst.steps
     ref, _ = solo(hip, good, Ug, 8)
     assert infg.status.err == hip.OK and np.array_equal(Ugo, ref)
+
+
+def test_burst_of_fresh_tiles_beyond_running_table(hip):
+    """One round admits more fresh tiles (priming table) than the running
+    subset holds and than the initial table capacity (1024 tiles): the
+    stepping table must stay intact while the priming table grows.  Five
+    Euler 256^2 requests with 2-row tiles are 640 tiles each; the cell cap
+    lets one instance run at a time."""
+    p = problem(hip, 0, True, 2, 256, 256, (T, T, T, T))
+    U0s = [states.state(0, 256, 256, seed=40 + k) for k in range(5)]
+    steps = [5, 7, 4, 6, 3]
+    with hip.Server(hbm_budget_bytes=1 << 30, max_active=8, steps_per_round=2, tile_rows=2,
+                    max_active_cells=70000, policy=0) as srv:
+        ids = [srv.submit(p, U, max_steps=n) for U, n in zip(U0s, steps)]
+        rounds = 0
+        while True:
+            a, q = srv.poll()
+            rounds += 1
+            if a == 0 and q == 0:
+                break
+        results = [srv.result(i) for i in ids]
+    print(f"[serve-burst] rounds={rounds}", flush=True)
+    for k, (U0, n) in enumerate(zip(U0s, steps)):
+        U, info = results[k]
+        ref, rst = solo(hip, p, U0, n)
+        assert info.status.err == hip.OK and info.status.steps == rst.steps == n
+        assert np.array_equal(U, ref), f"request {k} differs from its solo run"

@@ -180,3 +180,173 @@ def test_weno4_fourth_order_polynomial_in_linear_regime(seed):
     ux, uy, uxy = stencil_from_field(lambda i, j: cubic_avg(i, j, a))
     co = recon(3, ux, uy, uxy)
     assert np.allclose(co, a, rtol=0, atol=1e-6 * np.max(np.abs(a)))
+
+
+# ---- CWENO4 (reconstruction.c:223-357) in the nonlinear-weight regime -------
+#
+# Every candidate polynomial is obtained here from the cell averages it must
+# reproduce (a linear solve on quadrature cell averages of the basis), and
+# every smoothness indicator is the Jiang-Shu sum over all derivatives
+# D^a, 1 <= |a| <= degree, of the cell integral of (D^a p)^2, evaluated by
+# Gauss quadrature of the fitted polynomial.  No constant of the closed-form
+# indicators (13/3, 7/6, 0.1, 39.05, 4.7) or of the candidate formulas
+# (41/60, 11/120, ...) enters the expected values.
+
+CELLS = [(0, 0), (-2, 0), (-1, 0), (1, 0), (2, 0), (0, -2), (0, -1), (0, 1), (0, 2),
+         (1, 1), (1, -1), (-1, 1), (-1, -1)]
+
+
+def cell_avg_matrix(nb):
+    """A[c, k]: average of basis k over cell CELLS[c] (exact: degree <= 3)."""
+    return np.array([[quad_avg(i, j, np.eye(nb)[k]) for k in range(nb)] for (i, j) in CELLS])
+
+
+def functional(terms):
+    """Linear functional on the vector of the 13 stencil averages."""
+    v = np.zeros(len(CELLS))
+    for cell, wgt in terms:
+        v[CELLS.index(cell)] += wgt
+    return v
+
+
+def avg(cell):
+    return functional([(cell, 1.0)])
+
+
+# The conditions that define each candidate: cell averages it reproduces and,
+# for the corner information, the combinations of corner averages it matches.
+R4_CONDITIONS = [
+    avg((0, 0)), avg((1, 0)), avg((-1, 0)), avg((0, 1)), avg((0, -1)),
+    functional([((2, 0), 1.0), ((-2, 0), -1.0)]),
+    functional([((0, 2), 1.0), ((0, -2), -1.0)]),
+    functional([((1, 1), 1.0), ((1, -1), -1.0), ((-1, 1), -1.0), ((-1, -1), 1.0)]),
+    functional([((1, 1), 1.0), ((-1, 1), 1.0), ((1, -1), -1.0), ((-1, -1), -1.0)]),
+    functional([((1, 1), 1.0), ((1, -1), 1.0), ((-1, 1), -1.0), ((-1, -1), -1.0)]),
+]
+R3_CONDITIONS = [
+    # central
+    [avg((0, 0)), avg((1, 0)), avg((-1, 0)), avg((0, 1)), avg((0, -1)),
+     functional([((1, 1), 1.0), ((1, -1), -1.0), ((-1, 1), -1.0), ((-1, -1), 1.0)])],
+    # left-biased in x
+    [avg((0, 0)), avg((-1, 0)), avg((-2, 0)), avg((0, 1)), avg((0, -1)),
+     functional([((-1, 1), 1.0), ((-1, -1), -1.0)])],
+    # right-biased in x
+    [avg((0, 0)), avg((1, 0)), avg((2, 0)), avg((0, 1)), avg((0, -1)),
+     functional([((1, 1), 1.0), ((1, -1), -1.0)])],
+    # biased downwards in y
+    [avg((0, 0)), avg((0, -1)), avg((0, -2)), avg((1, 0)), avg((-1, 0)),
+     functional([((1, -1), 1.0), ((-1, -1), -1.0)])],
+    # biased upwards in y
+    [avg((0, 0)), avg((0, 1)), avg((0, 2)), avg((1, 0)), avg((-1, 0)),
+     functional([((1, 1), 1.0), ((-1, 1), -1.0)])],
+]
+
+
+def fit(conditions, nb, data):
+    A = cell_avg_matrix(nb)
+    M = np.array([c @ A for c in conditions])
+    return np.linalg.solve(M, np.array([c @ data for c in conditions]))
+
+
+_MONO = None
+
+
+def monomial_coeffs(a):
+    """Coefficients c[m, n] of x^m y^n (m + n <= 3) of sum_k a_k basis_k,
+    by exact interpolation of the basis at a 4 x 4 point set."""
+    global _MONO
+    if _MONO is None:
+        pts = [(x, y) for x in (-0.4, -0.1, 0.2, 0.45) for y in (-0.35, -0.05, 0.15, 0.4)]
+        mons = [(m, n) for m in range(4) for n in range(4) if m + n <= 3]
+        V = np.array([[x ** m * y ** n for (m, n) in mons] for (x, y) in pts])
+        B = np.array([[basis(x, y, k) for k in range(10)] for (x, y) in pts])
+        sol = np.linalg.lstsq(V, B, rcond=None)[0]  # (10 monomials) x (10 basis)
+        _MONO = (mons, sol)
+    mons, sol = _MONO
+    c = np.zeros((4, 4))
+    for r, (m, n) in enumerate(mons):
+        c[m, n] = sol[r, : len(a)] @ a
+    return c
+
+
+def jiang_shu(a, degree):
+    """sum over 1 <= |alpha| <= degree of the cell integral of (D^alpha p)^2."""
+    P = np.polynomial.polynomial
+    c = monomial_coeffs(a)
+    total = 0.0
+    for ax in range(degree + 1):
+        for ay in range(degree + 1 - ax):
+            if ax + ay == 0:
+                continue
+            d = P.polyder(P.polyder(c, ax, axis=0), ay, axis=1) if ay else P.polyder(c, ax, axis=0)
+            total += sum(wx * wy * P.polyval2d(x, y, d) ** 2 for x, wx in zip(GX, GW) for y, wy in zip(GX, GW))
+    return total
+
+
+def cweno4_expected(ux, uy, uxy):
+    data = np.zeros(len(CELLS))
+    vals = {(i, 0): ux[i + 2] for i in range(-2, 3)}
+    vals.update({(0, j): uy[j + 2] for j in range(-2, 3)})
+    vals.update({(1, 1): uxy[1], (1, -1): uxy[2], (-1, 1): uxy[3], (-1, -1): uxy[4]})
+    for c, cell in enumerate(CELLS):
+        data[c] = vals[cell]
+    a4 = fit(R4_CONDITIONS, 10, data)
+    a3 = [np.concatenate([fit(cond, 6, data), np.zeros(4)]) for cond in R3_CONDITIONS]
+    g_hi = g_lo = 0.85
+    gam4 = g_hi
+    gam3 = np.array([(1 - g_hi) * g_lo] + [0.25 * (1 - g_hi) * (1 - g_lo)] * 4)
+    w4 = gam4 / (jiang_shu(a4, 3) + EPS) ** 4
+    w3 = np.array([gam3[k] / (jiang_shu(a3[k], 2) + EPS) ** 4 for k in range(5)])
+    s = w4 + w3.sum()
+    w4, w3 = w4 / s, w3 / s
+    opt = a4 - sum(gam3[k] * a3[k] for k in range(5))
+    return (w4 / gam4) * opt + sum(w3[k] * a3[k] for k in range(5)), a4, a3
+
+
+def test_cweno4_candidates_reproduce_cubics():
+    """Sanity of the fitting: on cubic cell averages the fourth-order
+    candidate is the cubic, each third-order one its quadratic part's fit."""
+    a = np.random.default_rng(7).normal(size=10)
+    ux, uy, uxy = stencil_from_field(lambda i, j: cubic_avg(i, j, a))
+    _, a4, _ = cweno4_expected(ux, uy, uxy)
+    assert np.allclose(a4, a, rtol=0, atol=1e-12 * np.max(np.abs(a)))
+    q = np.zeros(10)
+    q[:6] = a[:6]
+    ux, uy, uxy = stencil_from_field(lambda i, j: cubic_avg(i, j, q))
+    _, _, a3 = cweno4_expected(ux, uy, uxy)
+    for cand in a3:
+        assert np.allclose(cand, q, rtol=0, atol=1e-12 * np.max(np.abs(q)))
+
+
+def test_jiang_shu_indicator_closed_forms():
+    """The quadrature indicator against hand integrals on single modes:
+    x -> 1, x^2 - 1/12 -> 13/3, xy -> 7/6, x(x^2 - 3/20) -> 39.06 (incl. its
+    cross term with x), y(x^2 - 1/12) -> 4.7."""
+    for k, want in [(1, 1.0), (3, 13.0 / 3.0), (5, 7.0 / 6.0), (6, 39.06), (8, 4.7), (9, 4.7), (7, 39.06)]:
+        a = np.zeros(10)
+        a[k] = 1.0
+        assert abs(jiang_shu(a, 3) - want) < 1e-13 * want
+
+
+@pytest.mark.parametrize("seed", range(10))
+@pytest.mark.parametrize("scale", [1.0, 1e-2, 1e2])
+def test_cweno4_nonlinear_weights(seed, scale):
+    """Data with a jump through the stencil (and smooth data of moderate
+    amplitude for odd seeds): weights far from linear, so every term of every
+    indicator moves the result."""
+    rng = np.random.default_rng(300 + seed)
+    vals = {}
+
+    def f(i, j):
+        if (i, j) not in vals:
+            if seed % 2:
+                vals[(i, j)] = scale * (np.sin(0.9 * i + 0.4 * j + seed) + 0.3 * np.cos(1.3 * j - 0.2 * i))
+            else:
+                vals[(i, j)] = scale * (rng.normal() + (2.5 if 0.7 * i - j > 0.4 else 0.0))
+        return vals[(i, j)]
+
+    ux, uy, uxy = stencil_from_field(f)
+    co = recon(3, ux, uy, uxy)
+    expect, _, _ = cweno4_expected(ux, uy, uxy)
+    tol = 1e-11 * max(np.max(np.abs(expect)), scale)
+    assert np.max(np.abs(co - expect)) <= tol, (co, expect)
