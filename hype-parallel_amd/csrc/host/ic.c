@@ -143,6 +143,142 @@ void hype_rankine_hugoniot(double gamma, double rho0, double p0, double M, doubl
     *u1 = W * (1.0 - rho0 / *rho1);
 }
 
+/* ---- the reference's Kapila test cases (kapila_5eqn.c:565-996) ----------
+ * Pointwise primitive states (rho1, rho2, u, v, p, alpha1), each with the
+ * region tests, comparison operators and constants of the reference's
+ * function: the IC is data the solver is validated on, so it must be the
+ * same state. */
+static void set6(double *V, double r1, double r2, double u, double v, double pr, double a1)
+{
+    V[0] = r1;
+    V[1] = r2;
+    V[2] = u;
+    V[3] = v;
+    V[4] = pr;
+    V[5] = a1;
+}
+
+static void kapila_case(int kind, double x, double y, double *V)
+{
+    const double eps = 1.0e-5;
+    switch (kind) {
+    case HYPE_IC_INTERFACE_ADVECTION: /* kapila_5eqn.c:572-594 */
+        set6(V, 1000.0, 1.2, 150.0, 0.0, 101325.0, (x < 0.5) ? 0.01 : 0.99);
+        break;
+    case HYPE_IC_AIR_HELIUM:          /* :606-644 */
+    case HYPE_IC_AIR_R22: {           /* :656-693 */
+        const double r2 = (kind == HYPE_IC_AIR_HELIUM) ? 0.25463 : 4.41540;
+        const double x0 = 0.245, y0 = 0.0455, R = 0.025;
+        if (x >= 0.275) set6(V, 1.92691, r2, -114.42, 0.0, 1.56980e5, 1.0 - eps);
+        else set6(V, 1.4, r2, 0.0, 0.0, 1.0e5, 1.0 - eps);
+        if ((x - x0) * (x - x0) + (y - y0) * (y - y0) <= R * R) set6(V, 1.4, r2, 0.0, 0.0, 1.0e5, eps);
+        break;
+    }
+    case HYPE_IC_RICHTMYER_MESHKOV: { /* :705-741 */
+        const double xs = 2.9 + 0.1 * sin(2.0 * M_PI * (y + 0.25));
+        if (x <= xs) set6(V, 1.0, 5.0, 0.0, 0.0, 1.0 / 1.4, eps);
+        else if (x > xs && x <= 3.2) set6(V, 1.0, 5.0, 0.0, 0.0, 1.0 / 1.4, 1.0 - eps);
+        else set6(V, 1.4112, 5.0, -0.3613, 0.0, 1.6272 / 1.4, 1.0 - eps);
+        break;
+    }
+    case HYPE_IC_UNDERWATER_EXPLOSION: { /* :753-805 */
+        const double x_0 = 0.0, y_0 = -0.3, r = 0.12;
+        if (y >= 0.0) set6(V, 1.0, 1.225e-03, 0.0, 0.0, 1.01325, eps);
+        else set6(V, 1.0, 1.225e-03, 0.0, 0.0, 1.01325, 1.0 - eps);
+        if ((x - x_0) * (x - x_0) + (y - y_0) * (y - y_0) <= r * r) set6(V, 1.0, 1.25, 0.0, 0.0, 1.0e4, eps);
+        if (y >= 0) V[5] = eps;
+        break;
+    }
+    case HYPE_IC_WATER_CYLINDER: {    /* :817-855 */
+        const double x0 = 0.04, y0 = 0.0, R0 = 1.1e-2;
+        if (x < 0.028) set6(V, 1.0, 3.7579, 574.57, 0.0, 6.6189e5, eps);
+        else set6(V, 1.0, 1.17, 0.0, 0.0, 1.01e5, eps);
+        if ((x - x0) * (x - x0) + (y - y0) * (y - y0) < R0 * R0) set6(V, 1000.0, 1.0, 0.0, 0.0, 1.01e5, 1.0 - eps);
+        break;
+    }
+    case HYPE_IC_AIR_CAVITY: {        /* :867-906 */
+        const double x0 = 6.0, y0 = 6.0, R = 3.0;
+        if (x > 11.4) set6(V, 1.325, 0.0012, -68.525, 0.0, 19153.0, 1.0 - eps);
+        else set6(V, 1.0, 0.0012, 0.0, 0.0, 1.0, 1.0 - eps);
+        if ((x - x0) * (x - x0) + (y - y0) * (y - y0) < R * R) set6(V, 1.0, 0.0012, 0.0, 0.0, 1.0, eps);
+        break;
+    }
+    case HYPE_IC_BUBBLE_COLLAPSE: {   /* :918-945 */
+        const double R0 = 1.0, x0 = 0.0, y0 = 0.0;
+        const double R = sqrt((x - x0) * (x - x0) + (y - y0) * (y - y0));
+        const double rhoi = 1.0e-2, pi = 1.0, alphai = 1.0e-5;
+        const double rhoo = 1.0, po = 100.0, alphao = 1.0 - 1.0e-5;
+        const double h = 10.0 / 500.0, smear = 3.0;
+        set6(V, rhoo, rhoi, 0.0, 0.0, 0.5 * ((pi + po) + (po - pi) * tanh((R - R0) / (smear * h))),
+             0.5 * ((alphai + alphao) + (alphao - alphai) * tanh((R - R0) / (smear * h))));
+        break;
+    }
+    case HYPE_IC_AIR_JET:             /* :957-972 */
+        set6(V, 997.0, 1.225, 0.0, 0.0, 1.013e5, 1.0 - eps);
+        break;
+    case HYPE_IC_WATER_JET:           /* :982-996 */
+        set6(V, 1.0, 1.0, 0.0, 0.0, 1.0e5, eps);
+        break;
+    default: memset(V, 0, 6 * sizeof(double));
+    }
+}
+
+/* The setting each test case's header comment states (kapila_5eqn.c:519-990):
+ * domain, boundary kinds (left, right, bottom, top), EOS constants, final
+ * time.  Where a header names PI_1 twice, the second is PI_2 (zero). */
+typedef struct {
+    int kind;
+    double x0, x1, y0, y1;
+    int bc[4];
+    double g1, g2, p1, p2, t_end;
+} case_setup;
+
+#define T_ HYPE_BC_TRANSMISSIVE
+#define R_ HYPE_BC_REFLECTIVE
+#define P_ HYPE_BC_PERIODIC
+#define I_ HYPE_BC_INLET
+static const case_setup CASES[] = {
+    {HYPE_IC_SMOOTH_VORTEX, -3.0, 3.0, -3.0, 3.0, {P_, P_, P_, P_}, 4.0, 1.4, 20.0, 0.0, 6.0},
+    {HYPE_IC_INTERFACE_ADVECTION, 0.0, 1.0, -0.05, 0.05, {T_, T_, T_, T_}, 4.4, 1.4, 6.0e8, 0.0, 0.26 / 150.0},
+    {HYPE_IC_AIR_HELIUM, 0.0, 0.356, 0.0, 0.089, {T_, T_, R_, R_}, 1.4, 1.648, 0.0, 0.0, 674.0e-6},
+    {HYPE_IC_AIR_R22, 0.0, 0.356, 0.0, 0.089, {T_, T_, R_, R_}, 1.4, 1.249, 0.0, 0.0, 417.0e-6},
+    {HYPE_IC_RICHTMYER_MESHKOV, 0.0, 4.0, -0.5, 0.5, {T_, T_, T_, T_}, 1.4, 1.093, 0.0, 0.0, 7.82},
+    {HYPE_IC_UNDERWATER_EXPLOSION, -2.0, 2.0, -1.5, 1.5, {T_, T_, R_, T_}, 4.4, 1.4, 6000.0, 0.0, 0.05},
+    {HYPE_IC_WATER_CYLINDER, 0.0, 0.12, -0.04, 0.04, {T_, T_, T_, T_}, 6.12, 1.4, 3.43e8, 0.0, 70.0e-6},
+    {HYPE_IC_AIR_CAVITY, 0.0, 12.0, 0.0, 12.0, {T_, T_, R_, R_}, 4.4, 1.4, 6000.0, 0.0, 0.045},
+    {HYPE_IC_BUBBLE_COLLAPSE, 0.0, 10.0, -2.5, 2.5, {R_, T_, T_, T_}, 4.4, 1.4, 6000.0, 0.0, 0.02},
+    {HYPE_IC_AIR_JET, 0.0, 30.0, -15.0, 15.0, {I_, T_, R_, R_}, 4.4, 1.4, 6000.0, 0.0, 1000.0},
+    {HYPE_IC_WATER_JET, 0.0, 1.0, -0.25, 0.25, {I_, T_, T_, T_}, 6.12, 1.4, 3.43e8, 0.0, 6.0e-6},
+};
+#undef T_
+#undef R_
+#undef P_
+#undef I_
+
+int hype_case_setup(int kind, hype_problem *p, double *t_end)
+{
+    size_t k;
+    if (!p) return HYPE_E_INVALID;
+    for (k = 0; k < sizeof(CASES) / sizeof(CASES[0]); ++k) {
+        const case_setup *c = &CASES[k];
+        int b;
+        if (c->kind != kind) continue;
+        p->system = HYPE_KAPILA;
+        p->x_min = c->x0;
+        p->x_max = c->x1;
+        p->y_min = c->y0;
+        p->y_max = c->y1;
+        for (b = 0; b < 4; ++b) p->bc[b] = c->bc[b];
+        p->gamma1 = c->g1;
+        p->gamma2 = c->g2;
+        p->pi1 = c->p1;
+        p->pi2 = c->p2;
+        if (t_end) *t_end = c->t_end;
+        return HYPE_OK;
+    }
+    return HYPE_E_INVALID;
+}
+
 static void point_state(const ic_ctx *C, double x, double y, double *V)
 {
     const hype_problem *p = C->p;
@@ -213,7 +349,11 @@ static void point_state(const ic_ctx *C, double x, double y, double *V)
         V[5] = alpha_min + 0.5 * (alpha_max - alpha_min) * erfc((r / R - 1.0) / k_eps);
         break;
     }
-    default: memset(V, 0, HYPE_MAXVAR * sizeof(double));
+    default:
+        if (C->kind >= HYPE_IC_INTERFACE_ADVECTION && C->kind <= HYPE_IC_WATER_JET)
+            kapila_case(C->kind, x, y, V);
+        else
+            memset(V, 0, HYPE_MAXVAR * sizeof(double));
     }
 }
 
@@ -225,16 +365,16 @@ int hype_initial_state(const hype_problem *p, int kind, const double *params, in
                                    -0.4530899229693320, 0.4530899229693320};
     static const double w_gp5[] = {0.28444444444444444, 0.23931433524968326, 0.23931433524968326,
                                    0.11846344252809456, 0.11846344252809456};
-    static const int need[] = {0, 18, 10, 13, 0};
+    static const int need[] = {0, 18, 10, 13, 0};  /* parameters of kinds 0..4; the test cases take none */
     ic_ctx C;
     int nv = 0, i, j, l, m, c;
     double h;
     HYPE_TRY(hype_check_problem(p, &nv));
-    if (!U || kind < 0 || kind > HYPE_IC_SMOOTH_VORTEX) return HYPE_E_INVALID;
+    if (!U || kind < 0 || kind > HYPE_IC_WATER_JET) return HYPE_E_INVALID;
     if (kind == HYPE_IC_UNIFORM && nparams < nv) return HYPE_E_INVALID;
-    if (kind != HYPE_IC_UNIFORM && nparams < need[kind]) return HYPE_E_INVALID;
+    if (kind <= HYPE_IC_SMOOTH_VORTEX && kind != HYPE_IC_UNIFORM && nparams < need[kind]) return HYPE_E_INVALID;
     if ((kind == HYPE_IC_QUADRANTS && p->system != HYPE_EULER) ||
-        ((kind == HYPE_IC_SHOCK_BUBBLE || kind == HYPE_IC_SMOOTH_VORTEX) && p->system != HYPE_KAPILA) ||
+        ((kind == HYPE_IC_SHOCK_BUBBLE || kind >= HYPE_IC_SMOOTH_VORTEX) && p->system != HYPE_KAPILA) ||
         (kind == HYPE_IC_WATER_AIR && p->system != HYPE_BAER_NUNZIATO))
         return HYPE_E_INVALID;
     memset(&C, 0, sizeof(C));

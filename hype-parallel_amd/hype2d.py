@@ -157,6 +157,7 @@ def lib():
     L.hype_initial_state.argtypes = [P(Problem), C.c_int, dp, C.c_int, C.c_ulonglong, C.c_int, dp]
     L.hype_rankine_hugoniot.argtypes = [C.c_double, C.c_double, C.c_double, C.c_double, dp, dp, dp]
     L.hype_rankine_hugoniot.restype = None
+    L.hype_case_setup.argtypes = [C.c_int, P(Problem), dp]
     L.hype_ctx_set_tiling.argtypes = [vp, C.c_int, C.c_int]
     L.hype_ctx_tiling.argtypes = [vp, P(C.c_int), P(C.c_int), P(C.c_int)]
     ip = P(C.c_int)
@@ -199,7 +200,7 @@ def exported_symbols():
         "hype_cons2prim", "hype_set_state", "hype_get_state", "hype_begin", "hype_advance",
         "hype_sync", "hype_dt_history", "hype_last_error", "hype_stream",
         "hype_ctx_set_tiling", "hype_ctx_tiling", "hype_advance_timed", "hype_initial_state",
-        "hype_rankine_hugoniot", "hype_batch_create", "hype_batch_destroy", "hype_batch_set_state",
+        "hype_rankine_hugoniot", "hype_case_setup", "hype_batch_create", "hype_batch_destroy", "hype_batch_set_state",
         "hype_batch_get_state", "hype_batch_begin", "hype_batch_advance", "hype_batch_advance_timed",
         "hype_batch_sync", "hype_batch_last_error", "hype_batch_tiling", "hype_batch_set_rows",
         "hype_batch_autotune", "hype_server_create", "hype_server_destroy", "hype_server_submit",
@@ -209,6 +210,9 @@ def exported_symbols():
 
 
 IC_UNIFORM, IC_QUADRANTS, IC_SHOCK_BUBBLE, IC_WATER_AIR, IC_SMOOTH_VORTEX = range(5)
+# the reference's Kapila test cases (kapila_5eqn.c:565-996), no parameters
+(IC_INTERFACE_ADVECTION, IC_AIR_HELIUM, IC_AIR_R22, IC_RICHTMYER_MESHKOV, IC_UNDERWATER_EXPLOSION,
+ IC_WATER_CYLINDER, IC_AIR_CAVITY, IC_BUBBLE_COLLAPSE, IC_AIR_JET, IC_WATER_JET) = range(5, 15)
 COORD_EXACT, COORD_DMDA = 0, 1
 
 
@@ -224,6 +228,17 @@ def initial_state(problem, kind, params=(), seed=0, coord=COORD_EXACT):
     if rc != OK:
         raise HypeError(rc, "hype_initial_state")
     return U
+
+
+def case_problem(kind, nx, ny, *, order=2, prim=True, cfl=0.5):
+    """Problem of a Kapila test case as its header states it
+    (hype_case_setup): domain, boundaries and EOS; returns (problem, t_end)."""
+    p = make_problem(KAPILA, nx, ny, order=order, prim=prim, cfl=cfl)
+    t_end = C.c_double()
+    rc = lib().hype_case_setup(kind, C.byref(p), C.byref(t_end))
+    if rc != OK:
+        raise HypeError(rc, "hype_case_setup")
+    return p, t_end.value
 
 
 def write_restart(path, U, block_size=None):

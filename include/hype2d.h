@@ -194,6 +194,19 @@ void *hype_stream(hype_ctx *c);
                                    nmodes (<= 8), amplitude, length; mode phases
                                    drawn from seed (water = phase 1, below) */
 #define HYPE_IC_SMOOTH_VORTEX 4 /* Kapila SmoothVortex (kapila_5eqn.c:527-562) */
+/* the reference's other Kapila test cases (no parameters) */
+#define HYPE_IC_INTERFACE_ADVECTION 5   /* kapila_5eqn.c:572 */
+#define HYPE_IC_AIR_HELIUM 6            /* kapila_5eqn.c:606 */
+#define HYPE_IC_AIR_R22 7               /* kapila_5eqn.c:656 */
+#define HYPE_IC_RICHTMYER_MESHKOV 8     /* kapila_5eqn.c:705 */
+#define HYPE_IC_UNDERWATER_EXPLOSION 9  /* kapila_5eqn.c:753 */
+#define HYPE_IC_WATER_CYLINDER 10       /* kapila_5eqn.c:817 */
+#define HYPE_IC_AIR_CAVITY 11           /* kapila_5eqn.c:867, the case
+                                           InitialCondition selects
+                                           (initialize_solution.c:14) */
+#define HYPE_IC_BUBBLE_COLLAPSE 12      /* kapila_5eqn.c:918 */
+#define HYPE_IC_AIR_JET 13              /* kapila_5eqn.c:957, left inlet */
+#define HYPE_IC_WATER_JET 14            /* kapila_5eqn.c:982, left inlet */
 
 #define HYPE_COORD_EXACT 0      /* cell centres x_min + (i+1/2) h */
 #define HYPE_COORD_DMDA 1       /* the centres DMDASetUniformCoordinates gives the
@@ -202,6 +215,13 @@ void *hype_stream(hype_ctx *c);
 int hype_initial_state(const hype_problem *p, int kind, const double *params,
                        int nparams, unsigned long long seed, int coord_mode,
                        double *U);
+
+/* The setting a Kapila test case's header comment states
+ * (kapila_5eqn.c:519-990): sets system, domain, boundary kinds and EOS
+ * constants of *p (grid, order, path, CFL and options are left to the
+ * caller; the grid must give square cells) and, if t_end is not NULL, the
+ * case's final time.  kind: HYPE_IC_SMOOTH_VORTEX .. HYPE_IC_WATER_JET. */
+int hype_case_setup(int kind, hype_problem *p, double *t_end);
 
 /* Post-shock state of a Mach-M shock moving +x into gas at rest
  * (Rankine-Hugoniot, ideal/stiffened gas with zero stiffness). */
