@@ -2,15 +2,21 @@
 """Benchmark of the HyPE-2D finite-volume RHS on one MI355X.
 
 A "step" is one full SSP-RK time step (all stages) of the workload, with the
-state resident in HBM.  Default workload: the Kapila 5-equation air-helium
-shock-bubble on 1024x512 cells, FP64, 500 steps (WENO3 + Rusanov + SSPRK(4,3),
-CFL 0.5).  Prints one JSON line with the throughput in Mcell-updates/s, the
-HBM roofline of the stage kernel and a CPU baseline (the FP64 C oracle on the
-host cores, bounded sample).
+states resident in HBM.  The default run (``--config all``) times the largest
+configuration that fits one GPU, the batched set: 64 independent Kapila
+air-helium shock-bubbles with sides log-uniform on [128, 1024], FP64, WENO3 +
+Rusanov + SSPRK(4,3), 400 steps each, advanced together by one persistent
+launch per RK stage (hype_batch).  Its JSON line carries the throughput in
+Mcell-updates/s, the HBM roofline of the stage kernel, the same instances
+solved one at a time, and a CPU baseline (the FP64 C oracle with MPI ranks
+on the host cores, bounded sample).  The three fixed grids are timed in the
+same run and reported under "configs", each with its own roofline and CPU
+baseline: Kapila 1024x512 (500 steps), Euler 256^2 (200), Baer-Nunziato
+2048x1024 (300).
 
-    python bench.py [--steps K] [--warmup W] [--config kapila|euler|bn]
-    python bench.py --config batched    # 64 uneven Kapila instances at once
-    python bench.py --config served     # Poisson request stream, continuous batching
+    python bench.py [--steps K] [--warmup W]            # batched + the three grids
+    python bench.py --config kapila|euler|bn|batched     # one configuration
+    python bench.py --config served [--requests N]       # Poisson request stream
 """
 import argparse
 import json
@@ -27,71 +33,10 @@ sys.path.insert(0, os.path.join(ROOT, "hype-parallel_amd"))
 import numpy as np  # noqa: E402
 
 import hype2d as H  # noqa: E402
+import workloads as WL  # noqa: E402
 
 METRIC = "Mcell-updates/s + % HBM peak; served p50/p99 latency vs one-at-a-time"
 HBM_PEAK_GBS = 8000.0
-
-
-def workload(name):
-    """(problem, initial state, steps, description, recons path)."""
-    if name == "kapila":
-        nx, ny = 1024, 512
-        h = 2.0 / nx
-        p = H.make_problem(H.KAPILA, nx, ny, order=2, prim=True, domain=(0.0, 2.0, 0.0, 1.0), cfl=0.5,
-                           bc=(H.BC_TRANSMISSIVE, H.BC_TRANSMISSIVE, H.BC_REFLECTIVE, H.BC_REFLECTIVE),
-                           gamma1=1.4, gamma2=1.648, pi1=0.0, pi2=0.0)
-        params = [1.22, 0.5, 1.0, 0.5, 0.25, 1e-6, 2.0 * h, 1.4, 0.25463, 1.0e5]
-        U0 = H.initial_state(p, H.IC_SHOCK_BUBBLE, params)
-        return p, U0, 500, "kapila_shock_bubble_1024x512"
-    if name == "euler":
-        nx = ny = 256
-        p = H.make_problem(H.EULER, nx, ny, order=2, prim=True, domain=(0.0, 1.0, 0.0, 1.0), cfl=0.5,
-                           bc=(H.BC_TRANSMISSIVE,) * 4, gamma1=1.4)
-        params = [0.5, 0.5, 1.5, 0.0, 0.0, 1.5, 0.5323, 1.206, 0.0, 0.3, 0.138, 1.206, 1.206, 0.029,
-                  0.5323, 0.0, 1.206, 0.3]
-        U0 = H.initial_state(p, H.IC_QUADRANTS, params)
-        return p, U0, 200, "euler_quadrants_256x256"
-    if name == "bn":
-        nx, ny = 2048, 1024
-        h = 2.0 / nx
-        p = H.make_problem(H.BAER_NUNZIATO, nx, ny, order=2, prim=False, domain=(0.0, 2.0, 0.0, 1.0),
-                           cfl=0.5, bc=(H.BC_TRANSMISSIVE,) * 4, gamma1=4.4, gamma2=1.4, pi1=6.0e8, pi2=0.0)
-        params = [0.5, 1e-6, 2.0 * h, 1.0, 0.3, 0.1, 1.0e9, 1.0e5, 1000.0, 1.0, 8, 0.005, 2.0]
-        U0 = H.initial_state(p, H.IC_WATER_AIR, params, seed=2024)
-        return p, U0, 300, "baer_nunziato_water_air_2048x1024"
-    raise SystemExit(f"unknown config {name}")
-
-
-def dt_estimate(p, U):
-    """Initial-dt estimate CFL*h/(2*max(|u|+c)) from cell averages (frozen
-    sound speeds); used only to turn a requested step count into an end time."""
-    g1, g2, p1, p2 = p.gamma1, p.gamma2, p.pi1, p.pi2
-    if p.system == H.EULER:
-        rho = U[..., 0]
-        u, v = U[..., 1] / rho, U[..., 2] / rho
-        pr = (g1 - 1.0) * (U[..., 3] - 0.5 * rho * (u * u + v * v))
-        c = np.sqrt(np.maximum(g1 * pr / rho, 0.0))
-        s = np.maximum(np.abs(u), np.abs(v)) + c
-    elif p.system == H.KAPILA:
-        rho = U[..., 0] + U[..., 1]
-        u, v, a1 = U[..., 2] / rho, U[..., 3] / rho, U[..., 5]
-        a2 = 1.0 - a1
-        e = U[..., 4] - 0.5 * rho * (u * u + v * v)
-        pr = (e - a1 * g1 * p1 / (g1 - 1.0) - a2 * g2 * p2 / (g2 - 1.0)) / (a1 / (g1 - 1.0) + a2 / (g2 - 1.0))
-        c2 = (U[..., 0] * g1 * (pr + p1) / np.maximum(U[..., 0] / a1, 1e-300)
-              + U[..., 1] * g2 * (pr + p2) / np.maximum(U[..., 1] / np.maximum(a2, 1e-300), 1e-300)) / rho
-        s = np.maximum(np.abs(u), np.abs(v)) + np.sqrt(np.maximum(c2, 0.0))
-    else:
-        phi = U[..., 8]
-        s = 0.0
-        for q, a, g, pi in ((U[..., 0:4], phi, g1, p1), (U[..., 4:8], 1.0 - phi, g2, p2)):
-            r = q[..., 0] / a
-            u, v = q[..., 1] / q[..., 0], q[..., 2] / q[..., 0]
-            pr = (q[..., 3] / q[..., 0] - 0.5 * (u * u + v * v)) * r * (g - 1.0) - g * pi
-            c = np.sqrt(np.maximum(g * (pr + pi) / r, 0.0))
-            s = np.maximum(s, np.maximum(np.abs(u), np.abs(v)) + c)
-    h = (p.x_max - p.x_min) / p.nx
-    return p.cfl * h / (2.0 * float(np.max(s)))
 
 
 def bytes_per_cell_step(nv, order):
@@ -110,10 +55,11 @@ def mpi_exec():
     return None
 
 
-def cpu_baseline_mpi(p, U0, ranks, budget_s):
+def oracle_mpi(p, U0, ranks, budget_s):
     """The FP64 C oracle with MPI ranks (oracle/orc_mpi_baseline.c: row
     blocks, width-3 halo exchange per RHS call, Allreduce(MIN) of dt), as the
-    reference runs its PETSc path.  Returns None when it cannot run here."""
+    reference runs its PETSc path.  Returns (ranks, steps, seconds) of the
+    timed steps, or None when it cannot run here."""
     exe = os.path.join(ROOT, "oracle", "lib", "orc_mpi_baseline")
     mpiexec = mpi_exec()
     if not os.path.exists(exe) or mpiexec is None or ranks < 2 or p.ny < 3 * ranks:
@@ -133,25 +79,24 @@ def cpu_baseline_mpi(p, U0, ranks, budget_s):
         words = r.stdout.split()
         if r.returncode != 0 or len(words) < 6 or words[0] != "ranks":
             return None
-        nr, k, sec = int(words[1]), int(words[3]), float(words[5])
+        return int(words[1]), int(words[3]), float(words[5])
     except (OSError, subprocess.SubprocessError, ValueError):
         return None
     finally:
         os.unlink(path)
-    rate = p.nx * p.ny * k / max(sec, 1e-9) / 1e6
-    return {"value": round(rate, 4), "unit": "Mcell-updates/s", "cores": nr, "kind": "port",
-            "sample": f"{k} SSP-RK steps of the same {p.nx}x{p.ny} workload after its priming call and "
-                      f"first step (C oracle, {nr} MPI ranks: row blocks with a width-3 halo exchange per "
-                      f"RHS call and an Allreduce(MIN) of dt)"}
 
 
-def cpu_baseline(p, U0, nv, budget_s=12.0):
-    """The FP64 C oracle on the same workload, bounded sample: with MPI ranks
-    on the host cores where MPI is installed, else the OpenMP build."""
-    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    res = cpu_baseline_mpi(p, U0, threads, budget_s)
-    if res is not None:
-        return res
+MPI_NOTE = "C oracle, {r} MPI ranks: row blocks with a width-3 halo exchange per RHS call and an Allreduce(MIN) of dt"
+
+
+def host_threads():
+    return int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+
+
+def oracle_openmp(p, U0, budget_s):
+    """The OpenMP build of the oracle (where no MPI installation is found):
+    (threads, steps, seconds), priming and allocation cancelled by
+    differencing a 1-step and a (k+1)-step run."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_py as orc
 
@@ -165,11 +110,53 @@ def cpu_baseline(p, U0, nv, budget_s=12.0):
     t0 = time.perf_counter()
     orc.solve(op, U0, k + 1, threaded=True)
     tk = time.perf_counter() - t0
-    rate = p.nx * p.ny * k / max(tk - t1, 1e-9) / 1e6
-    return {"value": round(rate, 4), "unit": "Mcell-updates/s", "cores": threads, "kind": "port",
+    return host_threads(), k, max(tk - t1, 1e-9)
+
+
+def cpu_baseline(p, U0, nv, budget_s=12.0):
+    """The FP64 C oracle on the same workload, bounded sample: with MPI ranks
+    on the host cores where MPI is installed, else the OpenMP build."""
+    threads = host_threads()
+    res = oracle_mpi(p, U0, threads, budget_s)
+    if res is not None:
+        nr, k, sec = res
+        return {"value": round(p.nx * p.ny * k / sec / 1e6, 4), "unit": "Mcell-updates/s", "cores": nr,
+                "kind": "port",
+                "sample": f"{k} SSP-RK steps of the same {p.nx}x{p.ny} workload after its priming call and "
+                          f"first step ({MPI_NOTE.format(r=nr)})"}
+    nt, k, sec = oracle_openmp(p, U0, budget_s)
+    return {"value": round(p.nx * p.ny * k / sec / 1e6, 4), "unit": "Mcell-updates/s", "cores": nt,
+            "kind": "port",
             "sample": f"{k} SSP-RK steps of the same {p.nx}x{p.ny} workload from its initial state "
-                      f"(C oracle, OpenMP, {threads} threads; priming and allocation cancelled by "
+                      f"(C oracle, OpenMP, {nt} threads; priming and allocation cancelled by "
                       f"differencing a 1-step and a {k + 1}-step run)"}
+
+
+def cpu_baseline_batched(insts, budget_s=4.0, nsample=4):
+    """The oracle on a sample of the batched set: the instances at evenly
+    spaced ranks of the size order (smallest and largest included), each
+    solved alone on all host cores for about budget_s seconds, as the
+    reference would run them one program after another.  Rate = sampled
+    cell-updates over the sampled time."""
+    order = sorted(range(len(insts)), key=lambda i: insts[i][0].nx * insts[i][0].ny)
+    pick = sorted({order[round(k * (len(order) - 1) / (nsample - 1))] for k in range(nsample)})
+    threads = host_threads()
+    cell_steps, sec, cores, how, sides = 0, 0.0, threads, None, []
+    for i in pick:
+        p, U0 = insts[i]
+        res = oracle_mpi(p, U0, threads, budget_s)
+        if res is not None:
+            cores, k, s = res
+            how = MPI_NOTE.format(r=cores)
+        else:
+            cores, k, s = oracle_openmp(p, U0, budget_s)
+            how = f"C oracle, OpenMP, {cores} threads"
+        cell_steps += p.nx * p.ny * k
+        sec += s
+        sides.append(f"#{i} {p.nx}^2 x {k} steps")
+    return {"value": round(cell_steps / sec / 1e6, 4), "unit": "Mcell-updates/s", "cores": cores, "kind": "port",
+            "sample": f"{len(pick)} of the {len(insts)} instances, each alone for about {budget_s:g} s of "
+                      f"SSP-RK steps after its priming call and first step ({', '.join(sides)}; {how})"}
 
 
 def profile_figure(workload_name, key):
@@ -228,33 +215,18 @@ def per_launch(roofline, launches):
     return roofline
 
 
-def kapila_bubble(n, mach, R, cx, cy):
-    """Square n x n Kapila shock-bubble on [0,1]^2 (air-helium, reflective
-    top and bottom), shock at x = 0.05 moving +x."""
-    h = 1.0 / n
-    p = H.make_problem(H.KAPILA, n, n, order=2, prim=True, domain=(0.0, 1.0, 0.0, 1.0), cfl=0.5,
-                       bc=(H.BC_TRANSMISSIVE, H.BC_TRANSMISSIVE, H.BC_REFLECTIVE, H.BC_REFLECTIVE),
-                       gamma1=1.4, gamma2=1.648)
-    U0 = H.initial_state(p, H.IC_SHOCK_BUBBLE, [mach, 0.05, cx, cy, R, 1e-6, 2.0 * h, 1.4, 0.25463, 1.0e5])
-    return p, U0
-
-
 def progress(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-BATCHED_WORKLOAD = "batched_kapila_64x_loguniform_128_1024"
-
-
-def run_batched(args):
-    """64 Kapila instances of uneven size advanced together (hype_batch), against
-    the same instances solved one after another (hype_ctx)."""
-    rng = np.random.default_rng(args.seed)
+def run_batched(args, with_cpu=True):
+    """The batched set advanced together (hype_batch), against the same
+    instances solved one after another (hype_ctx, device-resident, graphs).
+    Returns the JSON record."""
     count = args.instances
-    steps = args.steps if args.steps is not None else 400
-    sides = np.exp(rng.uniform(np.log(128), np.log(1024), count)).astype(int)
-    insts = [kapila_bubble(int(n), rng.uniform(1.1, 2.0), rng.uniform(0.1, 0.3),
-                           0.5 + rng.uniform(-0.1, 0.1), 0.5 + rng.uniform(-0.1, 0.1)) for n in sides]
+    steps = args.steps if args.steps is not None else WL.BATCHED_STEPS
+    insts = WL.batched(args.seed, count)
+    wname = WL.batched_name(count)
     cells = sum(p.nx * p.ny for p, _ in insts)
     progress(f"batched: {count} instances, {cells} cells, {steps} steps")
     with H.Batch([p for p, _ in insts]) as b:
@@ -274,7 +246,6 @@ def run_batched(args):
     if any(s.steps != steps for s in st):
         raise SystemExit("batched: not every instance ran all steps")
     progress(f"batched: {ms:.1f} ms; now one at a time")
-    # one at a time: each instance on its own context, device-resident, graphs
     solo_ms = 0.0
     identical = True
     for i, (p, U0) in enumerate(insts):
@@ -288,48 +259,28 @@ def run_batched(args):
                 identical &= bool(np.array_equal(ctx.get_state(), check[i]))
     sec = ms / 1e3
     value = cells * steps / sec / 1e6
+    roof = per_launch(aggregate_roofline({(6, 2): cells * steps}, sec, wname), 4 * steps)
+    roof["kernel"] = "stage_kernel (Kapila, prim, N=2, split tiles; persistent tile queue)"
+    roof["avg_launch_us"] = round(sec / (4 * steps) * 1e6, 3)
     out = {
         "metric": METRIC, "value": round(value, 3), "unit": "Mcell-updates/s", "n_gpus": 1, "steps": steps,
         "warmup": args.warmup, "ms_per_step": round(ms / steps, 5), "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-        "config": {"workload": f"batched_kapila_{count}x_loguniform_128_1024", "instances": count,
-                   "cells": cells, "order": 2, "seed": args.seed, "tile_rows": rows, "tiles": ntiles,
-                   "variants": groups},
-        "roofline": per_launch(aggregate_roofline({(6, 2): cells * steps}, sec, BATCHED_WORKLOAD), 4 * steps),
-        "valu_roofline": valu_roofline(BATCHED_WORKLOAD, sec / (4 * steps)),
+        "config": {"workload": wname, "instances": count, "cells": cells, "nvar": 6, "order": 2,
+                   "path": "RHSFunctionPrim", "scheme": "WENO3+Rusanov+SSPRK(4,3)", "cfl": 0.5,
+                   "seed": args.seed, "tile_rows": rows, "tiles": ntiles, "variants": groups,
+                   "graph": True, "kernel_build": "exact"},
+        "roofline": roof,
+        "valu_roofline": valu_roofline(wname, sec / (4 * steps)),
         "one_at_a_time": {"value": round(cells * steps / (solo_ms / 1e3) / 1e6, 3), "unit": "Mcell-updates/s",
                           "ms": round(solo_ms, 3)},
         "speedup_vs_one_at_a_time": round(solo_ms / ms, 3),
         "identical_to_solo": identical, "checked_instances": len(check),
     }
-    print(json.dumps(out))
-
-
-def served_trace(rng, count, rate):
-    """Poisson arrivals; mix 50% Euler / 35% Kapila / 15% Baer-Nunziato; side
-    log-uniform on [64, 2048]; 100-2000 steps turned into an end time."""
-    t = np.cumsum(rng.exponential(1.0 / rate, count))
-    reqs = []
-    for k in range(count):
-        n = int(np.exp(rng.uniform(np.log(64), np.log(2048))))
-        steps = int(rng.integers(100, 2001))
-        u = rng.uniform()
-        h = 1.0 / n
-        if u < 0.5:
-            p = H.make_problem(H.EULER, n, n, order=2, prim=True, domain=(0.0, 1.0, 0.0, 1.0), cfl=0.5,
-                               bc=(H.BC_TRANSMISSIVE,) * 4, gamma1=1.4)
-            U0 = H.initial_state(p, H.IC_QUADRANTS, [0.5, 0.5, 1.5, 0.0, 0.0, 1.5, 0.5323, 1.206, 0.0, 0.3,
-                                                     0.138, 1.206, 1.206, 0.029, 0.5323, 0.0, 1.206, 0.3])
-        elif u < 0.85:
-            p, U0 = kapila_bubble(n, 1.22, 0.25, 0.5, 0.5)
-        else:
-            p = H.make_problem(H.BAER_NUNZIATO, n, n, order=2, prim=False, domain=(0.0, 1.0, 0.0, 1.0),
-                               cfl=0.5, bc=(H.BC_TRANSMISSIVE,) * 4, gamma1=4.4, gamma2=1.4, pi1=6.0e8)
-            U0 = H.initial_state(p, H.IC_WATER_AIR, [0.5, 1e-6, 2.0 * h, 0.5, 0.3, 0.1, 1.0e9, 1.0e5, 1000.0,
-                                                     1.0, 8, 0.005, 1.0], seed=int(rng.integers(1 << 62)))
-        t_end = steps * dt_estimate(p, U0)
-        reqs.append((float(t[k]), p, U0, t_end, 2 * steps))
-    return reqs
+    if with_cpu and not args.no_cpu:
+        progress("batched: CPU baseline")
+        out["cpu_baseline"] = cpu_baseline_batched(insts)
+    return out
 
 
 def serve_trace(reqs, policy=H.SERVE_FIFO, active_cells=0):
@@ -369,7 +320,7 @@ def run_served(args):
     batching (hype_server); time-to-solution p50/p99 and aggregate throughput,
     against the same requests served one at a time in arrival order."""
     rng = np.random.default_rng(args.seed)
-    reqs = served_trace(rng, args.requests, args.rate)
+    reqs = WL.served_trace(rng, args.requests, args.rate)
     progress(f"served: trace of {len(reqs)} requests built")
     res, base, makespan = serve_trace(reqs, POLICIES[args.policy], args.active_cells)
     tts = np.array([inf.t_done - base - r[0] for (_, inf), r in zip(res, reqs)])
@@ -410,7 +361,7 @@ def run_served(args):
         "metric": METRIC, "value": round(value, 3), "unit": "Mcell-updates/s", "n_gpus": 1,
         "steps": int(sum(inf.status.steps for _, inf in res)), "warmup": 3, "ms_per_step": None,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-        "config": {"workload": f"served_poisson_{args.rate:g}rps_{args.requests}req_mix_E50_K35_B15_side64_2048",
+        "config": {"workload": WL.served_name(args.rate, args.requests),
                    "requests": args.requests, "rate_per_s": args.rate, "seed": args.seed,
                    "policy": args.policy, "max_active_cells": args.active_cells,
                    "cell_steps": int(cell_steps)},
@@ -424,7 +375,7 @@ def run_served(args):
         "identical_to_solo": identical,
         "inadmissible_requests": int(sum(1 for _, inf in res if inf.status.err)),
     }
-    print(json.dumps(out))
+    return out
 
 
 def calibrate_copy(mib):
@@ -496,12 +447,74 @@ def timed_run(p, U0, steps, args, pcie=True):
     return ms, st, U_end, tiling, rhs_s
 
 
+def run_single(name, args, steps=None, fma=True, pcie=True, with_cpu=True):
+    """One fixed grid: device-resident SSP-RK steps timed with HIP events on
+    the context's stream.  Returns the JSON record."""
+    p, U0, steps_default, wname = WL.single(name)
+    steps = steps if steps is not None else steps_default
+    nv = H.NVAR[p.system]
+    cells = p.nx * p.ny
+    progress(f"{wname}: {steps} steps")
+    ms, st, U_end, (hrows, ntiles, grid), rhs_s = timed_run(p, U0, steps, args, pcie=pcie)
+    if st.steps != steps:
+        raise SystemExit(f"{wname}: only {st.steps} of {steps} steps ran")
+    fma_variant = None
+    if fma and not args.no_fma:
+        # the opt-in FMA-contracted build (HYPE_OPT_FMA) on the same run,
+        # reported beside the exact build, which is the headline
+        p_fma = H.Problem.from_buffer_copy(p)
+        p_fma.options = H.OPT_FMA
+        ms_f, st_f, U_f, _, _ = timed_run(p_fma, U0, steps, args, pcie=False)
+        fma_variant = {"value": round(cells * steps / (ms_f / 1e3) / 1e6, 3),
+                       "max_rel_dev_vs_exact": rel_dev(U_f, U_end, p.system),
+                       "final_time_rel_dev": abs(st_f.t - st.t) / st.t,
+                       "note": "HYPE_OPT_FMA: same kernels built with FMA contraction; not bitwise"}
+    sec = ms / 1e3
+    mcups = cells * steps / sec / 1e6
+    nstage = {0: 1, 1: 2, 2: 4, 3: 10}[p.order]
+    launches = steps * nstage
+    avg_launch_s = sec / launches
+    bytes_launch = cells * bytes_per_cell_step(nv, p.order) / nstage
+    achieved = bytes_launch / avg_launch_s / 1e9
+    out = {
+        "metric": METRIC, "value": round(mcups, 3), "unit": "Mcell-updates/s", "n_gpus": 1, "steps": steps,
+        "warmup": args.warmup, "ms_per_step": round(ms / steps, 5), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": wname, "cells": cells, "nvar": nv, "order": p.order,
+                   "path": "RHSFunctionPrim" if p.recons_prim else "RHSFunction",
+                   "scheme": "WENO3+Rusanov+SSPRK(4,3)" if p.order == 2 else f"order{p.order}",
+                   "cfl": p.cfl, "tile_rows": hrows, "tiles": ntiles, "graph": bool(args.graph),
+                   "kernel_build": "exact"},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 5),
+                     "traffic": load_traffic(wname),
+                     "kernel": "stage_kernel", "avg_launch_us": round(avg_launch_s * 1e6, 3),
+                     "algorithmic_bytes_per_launch": int(bytes_launch)},
+        "valu_roofline": valu_roofline(wname, avg_launch_s),
+        "final_time": st.t,
+    }
+    if rhs_s is not None:
+        out["pcie_inclusive"] = {"rhs_call_ms": round(rhs_s * 1e3, 4), "nstage": nstage,
+                                 "equivalent_value": round(cells / (rhs_s * nstage) / 1e6, 3),
+                                 "note": "hype_rhs on host buffers (upload U, download RHS), per RK stage"}
+    if fma_variant is not None:
+        out["fma_variant"] = fma_variant
+    if with_cpu and not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(p, U0, nv)
+    return out
+
+
+SUB_KEYS = ("value", "unit", "steps", "ms_per_step", "config", "roofline", "valu_roofline", "final_time",
+            "pcie_inclusive", "fma_variant", "cpu_baseline")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=None)
+    ap.add_argument("--steps", type=int, default=None,
+                    help="timed steps of the headline configuration (default: its own step count)")
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--config", default="kapila")
+    ap.add_argument("--config", default="all", choices=["all", "batched", "kapila", "euler", "bn", "served"])
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--graph", type=int, default=1)
     ap.add_argument("--hrows", type=int, default=0)
@@ -527,71 +540,19 @@ def main():
                                    "a single grid would split into blocks with one halo exchange per RK "
                                    "stage, not built here), so --gpus must be 1"}))
         sys.exit(2)
-    if args.config == "batched":
-        run_batched(args)
-        return
     if args.config == "served":
-        run_served(args)
-        return
-
-    p, U0, steps_default, wname = workload(args.config)
-    steps = args.steps if args.steps is not None else steps_default
-    nv = H.NVAR[p.system]
-    cells = p.nx * p.ny
-    ms, st, U_end, (hrows, ntiles, grid), rhs_s = timed_run(p, U0, steps, args)
-    fma_variant = None
-    if not args.no_fma:
-        # the opt-in FMA-contracted build (HYPE_OPT_FMA) on the same run,
-        # reported beside the exact build, which is the headline
-        p_fma = H.Problem.from_buffer_copy(p)
-        p_fma.options = H.OPT_FMA
-        ms_f, st_f, U_f, _, _ = timed_run(p_fma, U0, steps, args, pcie=False)
-        fma_variant = {"value": round(cells * steps / (ms_f / 1e3) / 1e6, 3),
-                       "max_rel_dev_vs_exact": rel_dev(U_f, U_end, p.system),
-                       "final_time_rel_dev": abs(st_f.t - st.t) / st.t,
-                       "note": "HYPE_OPT_FMA: same kernels built with FMA contraction; not bitwise"}
-    if st.steps != steps:
-        raise SystemExit(f"only {st.steps} of {steps} steps ran")
-    sec = ms / 1e3
-    mcups = cells * steps / sec / 1e6
-    nstage = {0: 1, 1: 2, 2: 4, 3: 10}[p.order]
-    launches = steps * nstage
-    avg_launch_s = sec / launches
-    bytes_launch = cells * bytes_per_cell_step(nv, p.order) / nstage
-    achieved = bytes_launch / avg_launch_s / 1e9
-    out = {
-        "metric": METRIC,
-        "value": round(mcups, 3),
-        "unit": "Mcell-updates/s",
-        "n_gpus": 1,
-        "steps": steps,
-        "warmup": args.warmup,
-        "ms_per_step": round(ms / steps, 5),
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f64",
-        "data": "synthetic",
-        "config": {"workload": wname, "cells": cells, "nvar": nv, "order": p.order,
-                   "path": "RHSFunctionPrim" if p.recons_prim else "RHSFunction",
-                   "scheme": "WENO3+Rusanov+SSPRK(4,3)" if p.order == 2 else f"order{p.order}",
-                   "cfl": p.cfl, "tile_rows": hrows, "tiles": ntiles, "graph": bool(args.graph),
-                   "kernel_build": "exact"},
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 5),
-                     "traffic": load_traffic(wname),
-                     "kernel": "stage_kernel", "avg_launch_us": round(avg_launch_s * 1e6, 3),
-                     "algorithmic_bytes_per_launch": int(bytes_launch)},
-        "valu_roofline": valu_roofline(wname, avg_launch_s),
-        "final_time": st.t,
-        "pcie_inclusive": {"rhs_call_ms": round(rhs_s * 1e3, 4), "nstage": nstage,
-                           "equivalent_value": round(cells / (rhs_s * nstage) / 1e6, 3),
-                           "note": "hype_rhs on host buffers (upload U, download RHS), per RK stage"},
-    }
-    if fma_variant is not None:
-        out["fma_variant"] = fma_variant
-    if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(p, U0, nv)
+        out = run_served(args)
+    elif args.config == "batched":
+        out = run_batched(args)
+    elif args.config == "all":
+        out = run_batched(args)
+        # the fixed grids at their own step counts, each with its roofline
+        out["configs"] = {}
+        for name in ("kapila", "euler", "bn"):
+            rec = run_single(name, args, fma=(name == "kapila"), pcie=(name == "kapila"))
+            out["configs"][rec["config"]["workload"]] = {k: rec[k] for k in SUB_KEYS if k in rec}
+    else:
+        out = run_single(args.config, args, steps=args.steps)
     print(json.dumps(out))
 
 

@@ -214,19 +214,62 @@ double hype_cell_coord(const hype_problem *p, int axis, int i, int coord_mode)
  * band of the row-major list (list[i] = band[i mod nxcd][i / nxcd]) keeps
  * vertically adjacent tiles, which read each other's three halo rows, on
  * the same L2. */
+void hype_xcd_order_into(const hype_tile *src, hype_tile *dst, int n, int nxcd)
+{
+    int x, i, start = 0;
+    if (nxcd < 2 || n <= nxcd) {
+        memcpy(dst, src, (size_t)n * sizeof(hype_tile));
+        return;
+    }
+    for (x = 0; x < nxcd; ++x) {
+        const int cnt = n / nxcd + (x < n % nxcd ? 1 : 0); /* slots i = x, x + nxcd, ... below n */
+        for (i = 0; i < cnt; ++i) dst[x + i * nxcd] = src[start + i];
+        start += cnt;
+    }
+}
+
 int hype_xcd_order(hype_tile *tiles, int n, int nxcd)
 {
     hype_tile *tmp;
-    int x, i, start = 0;
     if (nxcd < 2 || n <= nxcd) return HYPE_OK;
     tmp = (hype_tile *)malloc((size_t)n * sizeof(hype_tile));
     if (!tmp) return HYPE_E_NOMEM;
     memcpy(tmp, tiles, (size_t)n * sizeof(hype_tile));
-    for (x = 0; x < nxcd; ++x) {
-        const int cnt = n / nxcd + (x < n % nxcd ? 1 : 0); /* slots i = x, x + nxcd, ... below n */
-        for (i = 0; i < cnt; ++i) tiles[x + i * nxcd] = tmp[start + i];
-        start += cnt;
-    }
+    hype_xcd_order_into(tmp, tiles, n, nxcd);
     free(tmp);
     return HYPE_OK;
+}
+
+int hype_pick_layout(const hype_problem *p, int *pair, int *nsy, int *slots)
+{
+    const int modes = hype_layouts(p, hyped_pair_modes(p->system));
+    int ncu = hyped_num_cus(), pr, found = 0;
+    double best = 1e300;
+    if (ncu < 1) ncu = 256;
+    for (pr = 0; pr <= 1; ++pr) {
+        hype_variant v;
+        int sl, occ, ns;
+        double cost;
+        if (!(modes & (1 << pr))) continue;
+        v.system = p->system;
+        v.prim = p->recons_prim;
+        v.order = p->order;
+        v.fma = (p->options & HYPE_OPT_FMA) != 0;
+        v.pair = pr;
+        v.inlet = p->bc[0] == HYPE_BC_INLET;
+        sl = ncu * hyped_blocks_per_cu(&v) * hyped_block_waves(&v); /* resident waves */
+        if (sl < 1) sl = 4 * ncu;
+        occ = (sl + 4 * ncu - 1) / (4 * ncu);
+        ns = hype_pick_segments(p->nx, p->ny, 4 * ncu, occ, 0, pr);
+        cost = hype_tile_cost((long)hype_strips(p->nx) * ns * hyped_block_waves(&v),
+                              hype_tile_rows((p->ny + ns - 1) / ns, pr), 4 * ncu, occ);
+        if (cost < best - 1e-9) {
+            best = cost;
+            *pair = pr;
+            *nsy = ns;
+            if (slots) *slots = sl;
+            found = 1;
+        }
+    }
+    return found ? HYPE_OK : HYPE_E_UNSUPPORTED;
 }

@@ -29,6 +29,11 @@ void hype_inlet_table(const hype_problem *p, double *out);
 /* tile layouts a problem may use (bit 0: one wave per tile, bit 1: split
  * tiles), given those its system's kernel is built for */
 int hype_layouts(const hype_problem *p, int built);
+/* tile layout (*pair: 1 split tiles on two-wave workgroups, 0 one wave per
+ * tile) and row segments of a single grid: the tile-shape model's cheapest
+ * among the layouts the system's kernel is built for; *slots (may be NULL)
+ * receives the resident waves of that kernel on the device */
+int hype_pick_layout(const hype_problem *p, int *pair, int *nsy, int *slots);
 /* row-iterations of one wave for a tile of h rows */
 double hype_tile_rows(int h, int pair);
 /* modelled time of a launch of `waves` waves of `per` row-iterations each */
@@ -41,8 +46,10 @@ int hype_make_tiles(int inst, int nx, int ny, int nsx, int nsy, hype_tile *tiles
 
 /* devices dispatch workgroups round-robin over this many XCDs (MI355X) */
 #define HYPE_NXCD 8
-/* reorder a row-major tile list so each XCD gets a contiguous band */
+/* reorder a row-major tile list so each XCD gets a contiguous band (in
+ * place, or from src into dst) */
 int hype_xcd_order(hype_tile *tiles, int n, int nxcd);
+void hype_xcd_order_into(const hype_tile *src, hype_tile *dst, int n, int nxcd);
 
 /* face-key bits of the packed error word for an nx x ny grid */
 int hype_key_bits(int nx, int ny);

@@ -72,43 +72,11 @@ int hype_ctx_create(const hype_problem *p, hype_ctx **out)
     c->nstages = hyped_num_stages(p->order);
     c->ncell = (size_t)p->nx * p->ny;
     c->nbytes = c->ncell * nv * sizeof(double);
-    {
-        /* tile layout and row split: the tile-shape model's cheapest among the
-         * layouts the system's kernel is built for (one wave per tile, or split
-         * tiles on two-wave workgroups) */
-        const int modes = hype_layouts(p, hyped_pair_modes(p->system));
-        int ncu = hyped_num_cus(), pr;
-        double best = 1e300;
-        if (ncu < 1) ncu = 256;
-        c->pair = -1;
-        for (pr = 0; pr <= 1; ++pr) {
-            hype_variant v;
-            int slots, occ, nsy;
-            double cost;
-            if (!(modes & (1 << pr))) continue;
-            v.system = p->system;
-            v.prim = p->recons_prim;
-            v.order = p->order;
-            v.fma = (p->options & HYPE_OPT_FMA) != 0;
-            v.pair = pr;
-            v.inlet = p->bc[0] == HYPE_BC_INLET;
-            slots = ncu * hyped_blocks_per_cu(&v) * hyped_block_waves(&v); /* resident waves */
-            if (slots < 1) slots = 4 * ncu;
-            occ = (slots + 4 * ncu - 1) / (4 * ncu);
-            nsy = hype_pick_segments(p->nx, p->ny, 4 * ncu, occ, 0, pr);
-            cost = hype_tile_cost((long)hype_strips(p->nx) * nsy * hyped_block_waves(&v),
-                                  hype_tile_rows((p->ny + nsy - 1) / nsy, pr), 4 * ncu, occ);
-            if (cost < best - 1e-9) {
-                best = cost;
-                c->pair = pr;
-                c->nsy = nsy;
-                c->slots = slots;
-            }
-        }
-        if (c->pair < 0) {
-            free(c);
-            return HYPE_E_UNSUPPORTED;
-        }
+    /* tile layout and row split: the tile-shape model's cheapest among the
+     * layouts the system's kernel is built for */
+    if (hype_pick_layout(p, &c->pair, &c->nsy, &c->slots) != HYPE_OK) {
+        free(c);
+        return HYPE_E_UNSUPPORTED;
     }
     c->ntiles = hype_strips(p->nx) * c->nsy;
     c->grid = 0;

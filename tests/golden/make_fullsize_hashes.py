@@ -21,14 +21,14 @@ sys.path.insert(0, os.path.join(ROOT, "tests"))
 
 import numpy as np  # noqa: E402
 
-import bench  # noqa: E402
+import workloads  # noqa: E402
 import oracle_py as orc  # noqa: E402
 
 OUT = os.path.join(HERE, "fullsize_sha256.json")
 
 
 def oracle_final(name):
-    p, U0, steps, wname = bench.workload(name)
+    p, U0, steps, wname = workloads.single(name)
     op = orc.problem(p.system, p.nx, p.ny, N=p.order, prim=bool(p.recons_prim),
                      domain=(p.x_min, p.x_max, p.y_min, p.y_max), cfl=p.cfl, bc=tuple(p.bc),
                      g1=p.gamma1, g2=p.gamma2, p1=p.pi1, p2=p.pi2, quirks=p.quirks)

@@ -14,7 +14,7 @@ import pytest
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 sys.path.insert(0, ROOT)
 
-import bench  # noqa: E402
+import workloads  # noqa: E402
 
 pytestmark = pytest.mark.gpu
 
@@ -25,7 +25,7 @@ with open(os.path.join(ROOT, "tests", "golden", "fullsize_sha256.json")) as f:
 @pytest.mark.parametrize("wname", sorted(GOLDEN))
 def test_full_size_run_is_bitwise_the_oracle(hip, wname):
     rec = GOLDEN[wname]
-    p, U0, steps, name = bench.workload(rec["config"])
+    p, U0, steps, name = workloads.single(rec["config"])
     assert name == wname and steps == rec["steps"]
     with hip.Context(p) as ctx:
         U, st = ctx.solve(U0, steps, flags=hip.TS_CLIP | hip.TS_GRAPH)
