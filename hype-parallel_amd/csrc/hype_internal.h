@@ -105,6 +105,17 @@ typedef struct hype_launch {
     unsigned char *fallback;   /* RHS mode: optional per-cell fallback mask */
 } hype_launch;
 
+/* device scratch of the allocation-free RHS (hype_rhs_noalloc, scratch.hip):
+ * the instance descriptor at offset 0, the time state at STATE_OFF, the tile
+ * table at TILES_OFF (at most MAX_TILES tiles), the left-inlet table at
+ * INLET_OFF up to the end */
+#define HYPE_SCRATCH_BYTES (1u << 20)
+#define HYPE_SCRATCH_STATE_OFF 256
+#define HYPE_SCRATCH_TILES_OFF 512
+#define HYPE_SCRATCH_MAX_TILES 16384
+#define HYPE_SCRATCH_INLET_OFF (HYPE_SCRATCH_TILES_OFF + HYPE_SCRATCH_MAX_TILES * sizeof(hype_tile))
+#define HYPE_SCRATCH_INLET_DOUBLES ((HYPE_SCRATCH_BYTES - HYPE_SCRATCH_INLET_OFF) / sizeof(double))
+
 /* number of stages of the scheme for order N */
 int hyped_num_stages(int order);
 /* stage s of the order-N scheme: ibuf = {in, out, xb, zb, op}, coef = {a, g,
@@ -137,6 +148,9 @@ int hyped_block_waves(const hype_variant *v);
  * tile, bit 1 split tiles */
 int hyped_pair_modes(int system);
 const char *hyped_last_error(void);
+/* this device's address of the scratch array and its size (looked up on the
+ * first call on each device, with that device current) */
+int hyped_scratch(void **ptr, size_t *bytes);
 
 /* HIP events on a stream */
 int hyped_event_create(void **ev);

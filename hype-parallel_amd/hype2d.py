@@ -17,7 +17,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "lib", "libhype2d.so")
 
-OK, E_INVALID, E_UNSUPPORTED, E_DEVICE, E_NOMEM, E_INADMISSIBLE, E_BUSY, E_NOTFOUND = range(8)
+OK, E_INVALID, E_UNSUPPORTED, E_DEVICE, E_NOMEM, E_INADMISSIBLE, E_BUSY, E_NOTFOUND, E_TOO_LARGE = range(9)
 EULER, KAPILA, BAER_NUNZIATO = 0, 1, 2
 BC_WALL, BC_INLET, BC_PERIODIC, BC_REFLECTIVE, BC_TRANSMISSIVE = range(5)
 QUIRK_BN_P2_IS_PHI = 0x1
@@ -142,6 +142,7 @@ def lib():
     L.hype_ctx_destroy.restype = None
     L.hype_rhs.argtypes = [vp, C.c_double, dp, dp, dp, C.c_int]
     L.hype_rhs_diag.argtypes = [vp, C.c_double, dp, dp, dp, P(C.c_ubyte), C.c_int]
+    L.hype_rhs_noalloc.argtypes = [P(Problem), C.c_double, dp, dp, dp, dp, vp]
     L.hype_cons2prim.argtypes = [vp, dp, dp, C.c_int]
     L.hype_set_state.argtypes = [vp, dp, C.c_int]
     L.hype_get_state.argtypes = [vp, dp, C.c_int]
@@ -196,7 +197,7 @@ def lib():
 def exported_symbols():
     """Names declared in include/hype2d.h (checked by the CPU tests)."""
     return [
-        "hype_nvar", "hype_ctx_create", "hype_ctx_destroy", "hype_rhs", "hype_rhs_diag",
+        "hype_nvar", "hype_ctx_create", "hype_ctx_destroy", "hype_rhs", "hype_rhs_diag", "hype_rhs_noalloc",
         "hype_cons2prim", "hype_set_state", "hype_get_state", "hype_begin", "hype_advance",
         "hype_sync", "hype_dt_history", "hype_last_error", "hype_stream",
         "hype_ctx_set_tiling", "hype_ctx_tiling", "hype_advance_timed", "hype_initial_state",
@@ -275,6 +276,22 @@ def rankine_hugoniot(gamma, rho0, p0, mach):
 
 def _dptr(a):
     return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def rhs_noalloc(problem, U_ptr, R_ptr, t=0.0, stream=None, allow_inadmissible=False):
+    """hype_rhs_noalloc on device buffers (addresses U_ptr, R_ptr): the RHS
+    without a context or any allocation.  Returns (dt, rc, (x, y) of the
+    first inadmissible face or None)."""
+    dt = C.c_double()
+    xy = (C.c_double * 2)()
+    rc = lib().hype_rhs_noalloc(C.byref(problem), t, C.cast(U_ptr, C.POINTER(C.c_double)),
+                                C.cast(R_ptr, C.POINTER(C.c_double)), C.byref(dt),
+                                C.cast(xy, C.POINTER(C.c_double)), stream)
+    if rc == OK:
+        return dt.value, rc, None
+    if rc == E_INADMISSIBLE and allow_inadmissible:
+        return dt.value, rc, (xy[0], xy[1])
+    raise HypeError(rc, "hype_rhs_noalloc")
 
 
 class Context:

@@ -34,6 +34,8 @@ extern "C" {
                                  212-222; baer_nunziato.c:91-113,281-303) */
 #define HYPE_E_BUSY 6         /* server: no slot or HBM for the request yet */
 #define HYPE_E_NOTFOUND 7     /* server: unknown request id */
+#define HYPE_E_TOO_LARGE 8    /* hype_rhs_noalloc: the grid's tables exceed the
+                                 library's fixed device scratch */
 
 /* ---- PDE systems (compile-time #define in hype.h:33-35, runtime here) -- */
 #define HYPE_EULER 0          /* (rho, rho u, rho v, E), nvar 4 (euler.c) */
@@ -128,6 +130,27 @@ int hype_rhs(hype_ctx *c, double t, const double *U, double *RHS,
  * back to minmod slopes (rhs_function_prim.c:393-421). */
 int hype_rhs_diag(hype_ctx *c, double t, const double *U, double *RHS,
                   double *dt_out, unsigned char *fallback, int mem);
+
+/* hype_rhs on caller-owned device buffers, without allocation.  No
+ * counterpart in the reference: it is for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  It creates no context and calls no
+ * device or host allocator; the instance descriptor, time state, tile table
+ * and left-inlet table live in a fixed device scratch of 1 MiB per GPU that
+ * the library holds until the process ends.  U and RHS: device pointers
+ * (nx*ny*nvar doubles, AoS [j][i][c]); stream: a hipStream_t, or NULL for
+ * the null stream.  The call waits for its stream and returns the same
+ * RHS bits, *dt_out and status as hype_rhs on the same problem; on
+ * HYPE_E_INADMISSIBLE, err_xy (may be NULL) receives the face coordinates
+ * hype_last_error would give.  Limits: the tile table holds 16384 tiles
+ * (a grid the tile-shape model would cut finer gets taller tiles, with the
+ * same results; nx > 62 * 16384 = 1015808 is HYPE_E_TOO_LARGE), and the
+ * left-inlet table (ny + 6) * nvar <= 81856 doubles (Kapila: ny <= 13636,
+ * else HYPE_E_TOO_LARGE).  Every call shares the one scratch of its GPU and
+ * leaves its contents behind; calls in one process are serialised by the
+ * library, and two processes each hold their own scratch. */
+int hype_rhs_noalloc(const hype_problem *p, double t, const double *U, double *RHS,
+                     double *dt_out, double *err_xy, void *stream);
 
 /* Primitive pass (ComputePrimitiveVariables, post_process.c:165-200). */
 int hype_cons2prim(hype_ctx *c, const double *U, double *W, int mem);
