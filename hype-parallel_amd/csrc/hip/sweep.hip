@@ -282,6 +282,12 @@ struct Tile {
       }
     }
     const auto Uin = gmem<const double>((mode == HYPE_MODE_STAGE) ? I->buf[sd.in] : (L.rhs_in ? L.rhs_in : I->buf[0]));
+    // the stage update's buffers, read from the descriptor once per tile
+    // rather than in every row iteration
+    const auto Out = gmem((mode == HYPE_MODE_STAGE) ? I->buf[sd.out] : (L.rhs_out ? L.rhs_out : I->buf[1]));
+    const auto Xb = gmem<const double>(I->buf[sd.xb]);
+    const auto Zb = gmem<const double>(I->buf[sd.zb]);
+    const auto Ab = gmem(I->buf[1]);
     const double cdt = dt / sd.bden;
 
     // ---- geometry ------------------------------------------------------
@@ -463,31 +469,50 @@ struct Tile {
         rhs[c] = ((S::HAS_NCP && S::DNZ[c]) ? nc[c] : 0.0) + t;
       }
       if (mode == HYPE_MODE_RHS) {
-        const auto out = gmem(L.rhs_out ? L.rhs_out : I->buf[1]);
 #pragma unroll
-        for (int c = 0; c < NV; ++c) out[idx + c] = rhs[c];
+        for (int c = 0; c < NV; ++c) Out[idx + c] = rhs[c];
       } else {
-        const auto out = gmem(I->buf[sd.out]);
+        // every operand of the record is loaded before the first store.  The
+        // compiler cannot prove the output distinct from the inputs (it is
+        // not, for the RK stages that update U in place), so loads
+        // interleaved with the stores would each wait for memory: six round
+        // trips per row instead of one
+        const auto out = Out;
         if (sd.op == 0) {
+          double u[NV];
 #pragma unroll
-          for (int c = 0; c < NV; ++c) out[idx + c] = Uin[idx + c] + cdt * rhs[c];
+          for (int c = 0; c < NV; ++c) u[c] = Uin[idx + c];
+#pragma unroll
+          for (int c = 0; c < NV; ++c) out[idx + c] = u[c] + cdt * rhs[c];
         } else if (sd.op == 2) {
           // RK104 after its fifth stage: w0 += dt/6 F (VecAXPY); w1 =
           // 1/25 U + 9/25 w0 (VecAXPBYPCZ, c = 0) into U; w0 = 15 w1 - 5 w0
           // (VecAXPBY) into A
-          const auto A = gmem(I->buf[1]);
+          const auto A = Ab;
+          double u[NV], o[NV];
 #pragma unroll
           for (int c = 0; c < NV; ++c) {
-            const double w0 = Uin[idx + c] + cdt * rhs[c];
-            const double w1 = sd.a * out[idx + c] + sd.g * w0;
+            u[c] = Uin[idx + c];
+            o[c] = out[idx + c];
+          }
+#pragma unroll
+          for (int c = 0; c < NV; ++c) {
+            const double w0 = u[c] + cdt * rhs[c];
+            const double w1 = sd.a * o[c] + sd.g * w0;
             out[idx + c] = w1;
             A[idx + c] = 15.0 * w1 + -5.0 * w0;
           }
         } else {
-          const auto X = gmem<const double>(I->buf[sd.xb]);
-          const auto Z = gmem<const double>(I->buf[sd.zb]);
+          const auto X = Xb;
+          const auto Z = Zb;
+          double x[NV], z[NV];
 #pragma unroll
-          for (int c = 0; c < NV; ++c) out[idx + c] = sd.a * X[idx + c] + cdt * rhs[c] + sd.g * Z[idx + c];
+          for (int c = 0; c < NV; ++c) {
+            x[c] = X[idx + c];
+            z[c] = Z[idx + c];
+          }
+#pragma unroll
+          for (int c = 0; c < NV; ++c) out[idx + c] = sd.a * x[c] + cdt * rhs[c] + sd.g * z[c];
         }
       }
     };
@@ -814,7 +839,9 @@ __global__ __launch_bounds__(PR ? 2 * HYPE_WAVE : HYPE_WAVE)
 
 template <class S, bool PRIM, int N, bool PR, bool IN>
 static hipError_t launch_variant(const hype_launch& L, hipStream_t s) {
-  const int grid = L.grid > 0 ? L.grid : L.ntiles;  // device-resident counts need grid > 0
+  // one workgroup per table entry (grid 0; with a device-resident count the
+  // workgroups past it return at once), or a persistent grid pulling tiles
+  const int grid = L.grid > 0 ? L.grid : L.ntiles;
   hipLaunchKernelGGL((stage_kernel<S, PRIM, N, PR, IN>), dim3(grid), dim3(HYPE_WAVE * Tile<S, PRIM, N, PR, IN>::WAVES), 0,
                      s, L);
   return hipGetLastError();
