@@ -248,7 +248,7 @@ def run_batched(args, with_cpu=True):
     progress(f"batched: {ms:.1f} ms; now one at a time")
     solo_ms = 0.0
     identical = True
-    for i, (p, U0) in enumerate(insts):
+    for i, (p, U0) in enumerate([] if args.no_solo else insts):
         with H.Context(p) as ctx:
             ctx.set_state(U0)
             ctx.begin(0.0, float("inf"), steps, H.TS_CLIP | H.TS_GRAPH)
@@ -260,7 +260,7 @@ def run_batched(args, with_cpu=True):
     sec = ms / 1e3
     value = cells * steps / sec / 1e6
     roof = per_launch(aggregate_roofline({(6, 2): cells * steps}, sec, wname), 4 * steps)
-    roof["kernel"] = "stage_kernel (Kapila, prim, N=2, split tiles; persistent tile queue)"
+    roof["kernel"] = "stage_kernel (Kapila, prim, N=2, split tiles, XCD-banded tile table)"
     roof["avg_launch_us"] = round(sec / (4 * steps) * 1e6, 3)
     out = {
         "metric": METRIC, "value": round(value, 3), "unit": "Mcell-updates/s", "n_gpus": 1, "steps": steps,
@@ -272,44 +272,90 @@ def run_batched(args, with_cpu=True):
                    "graph": True, "kernel_build": "exact"},
         "roofline": roof,
         "valu_roofline": valu_roofline(wname, sec / (4 * steps)),
-        "one_at_a_time": {"value": round(cells * steps / (solo_ms / 1e3) / 1e6, 3), "unit": "Mcell-updates/s",
-                          "ms": round(solo_ms, 3)},
-        "speedup_vs_one_at_a_time": round(solo_ms / ms, 3),
-        "identical_to_solo": identical, "checked_instances": len(check),
     }
+    if not args.no_solo:
+        out["one_at_a_time"] = {"value": round(cells * steps / (solo_ms / 1e3) / 1e6, 3),
+                                "unit": "Mcell-updates/s", "ms": round(solo_ms, 3)}
+        out["speedup_vs_one_at_a_time"] = round(solo_ms / ms, 3)
+        out["identical_to_solo"] = identical
+        out["checked_instances"] = len(check)
     if with_cpu and not args.no_cpu:
         progress("batched: CPU baseline")
         out["cpu_baseline"] = cpu_baseline_batched(insts)
     return out
 
 
-def serve_trace(reqs, policy=H.SERVE_FIFO, active_cells=0):
-    """Replay a trace through one hype_server in real time; returns the
-    results (U, info) in submission order, the server clock at the first
-    arrival and the makespan."""
+class RequestBuilder:
+    """Initial states of the stream's requests, built ahead of their arrival
+    on host threads (the IC quadrature runs in C and releases the GIL), at
+    most `ahead` requests in advance so host memory stays bounded."""
+
+    def __init__(self, draws, ahead=48, workers=8):
+        from concurrent.futures import ThreadPoolExecutor
+
+        self.draws = draws
+        self.ahead = ahead
+        self.pool = ThreadPoolExecutor(max_workers=workers)
+        self.fut = {}
+        self.next = 0
+
+    def _fill(self, k):
+        while self.next < len(self.draws) and self.next < k + self.ahead:
+            _, system, n, steps, seed = self.draws[self.next]
+            self.fut[self.next] = self.pool.submit(WL.served_request, system, n, steps, seed)
+            self.next += 1
+
+    def get(self, k):
+        """(problem, U0, t_end, cap) of request k (blocks until built)."""
+        self._fill(k)
+        return self.fut.pop(k).result()
+
+    def close(self):
+        self.pool.shutdown(wait=True)
+
+
+def digest(U):
+    import hashlib
+
+    return hashlib.sha256(np.ascontiguousarray(U, dtype=np.float64).tobytes()).hexdigest()
+
+
+def serve_trace(draws, policy=H.SERVE_FIFO, active_cells=0):
+    """Replay a stream through one hype_server in real time.  Returns per
+    request (problem, t_end, cap, info, result digest) in submission order,
+    and the makespan; results are hashed and released as they complete."""
+    builder = RequestBuilder(draws)
+    out = [None] * len(draws)
     with H.Server(policy=policy, max_active_cells=active_cells) as srv:
         # warm-up: a few tiny requests (kernel loading, graph capture)
-        for _, p, U0, _, _ in reqs[:3]:
+        for k in range(min(3, len(draws))):
+            _, system, n, steps, seed = draws[k]
+            p, U0, _, _ = WL.served_request(system, min(n, 64), 2, seed)
             srv.submit(p, U0, max_steps=2)
         srv.drain()
         base = srv.now()
-        ids = []
+        pending = {}
         k = 0
         last = 0.0
-        while k < len(reqs) or srv.poll() != (0, 0):
+        while k < len(draws) or pending:
             now = srv.now() - base
             if now - last > 10.0:
                 last = now
-                progress(f"served: t={now:.1f}s submitted={k}")
-            while k < len(reqs) and reqs[k][0] <= now:
-                _, p, U0, t_end, cap = reqs[k]
-                ids.append(srv.submit(p, U0, t_end=t_end, max_steps=cap))
+                progress(f"served: t={now:.1f}s submitted={k} running/queued={len(pending)}")
+            while k < len(draws) and draws[k][0] <= now:
+                p, U0, t_end, cap = builder.get(k)
+                pending[srv.submit(p, U0, t_end=t_end, max_steps=cap)] = (k, p, t_end, cap)
                 k += 1
-            if k < len(reqs) and srv.poll() == (0, 0):
-                time.sleep(min(max(reqs[k][0] - (srv.now() - base), 0.0), 0.01))
+            a, q = srv.poll()
+            for rid in [r for r in pending if srv.info(r).state == H.REQ_DONE]:
+                j, p, t_end, cap = pending.pop(rid)
+                U, inf = srv.result(rid)
+                out[j] = (p, t_end, cap, inf, digest(U))
+            if (a, q) == (0, 0) and k < len(draws):
+                time.sleep(min(max(draws[k][0] - (srv.now() - base), 0.0), 0.01))
         makespan = srv.now() - base
-        res = [srv.result(i) for i in ids]
-    return res, base, makespan
+    builder.close()
+    return out, base, makespan
 
 
 POLICIES = {"fifo": H.SERVE_FIFO, "shortest": H.SERVE_SHORTEST}
@@ -320,23 +366,40 @@ def run_served(args):
     batching (hype_server); time-to-solution p50/p99 and aggregate throughput,
     against the same requests served one at a time in arrival order."""
     rng = np.random.default_rng(args.seed)
-    reqs = WL.served_trace(rng, args.requests, args.rate)
-    progress(f"served: trace of {len(reqs)} requests built")
-    res, base, makespan = serve_trace(reqs, POLICIES[args.policy], args.active_cells)
-    tts = np.array([inf.t_done - base - r[0] for (_, inf), r in zip(res, reqs)])
-    cell_steps = sum(inf.status.steps * r[1].nx * r[1].ny for (_, inf), r in zip(res, reqs))
+    draws = WL.served_draws(rng, args.requests, args.rate)
+    progress(f"served: stream of {len(draws)} requests drawn")
+    res, base, makespan = serve_trace(draws, POLICIES[args.policy], args.active_cells)
+    tts = np.array([inf.t_done - base - d[0] for (_, _, _, inf, _), d in zip(res, draws)])
+    cell_steps = sum(inf.status.steps * p.nx * p.ny for p, _, _, inf, _ in res)
     variants = {}
-    for (_, inf), r in zip(res, reqs):
-        key = (H.NVAR[r[1].system], 2)
-        variants[key] = variants.get(key, 0) + inf.status.steps * r[1].nx * r[1].ny
+    for p, _, _, inf, _ in res:
+        key = (H.NVAR[p.system], 2)
+        variants[key] = variants.get(key, 0) + inf.status.steps * p.nx * p.ny
+    out = {
+        "metric": METRIC, "value": round(cell_steps / makespan / 1e6, 3), "unit": "Mcell-updates/s", "n_gpus": 1,
+        "steps": int(sum(inf.status.steps for _, _, _, inf, _ in res)), "warmup": 3, "ms_per_step": None,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": WL.served_name(args.rate, args.requests),
+                   "requests": args.requests, "rate_per_s": args.rate, "seed": args.seed,
+                   "policy": args.policy, "max_active_cells": args.active_cells,
+                   "cell_steps": int(cell_steps)},
+        "roofline": aggregate_roofline(variants, makespan),
+        "latency_s": {"p50": round(float(np.percentile(tts, 50)), 4), "p99": round(float(np.percentile(tts, 99)), 4),
+                      "makespan": round(makespan, 3)},
+        "inadmissible_requests": int(sum(1 for _, _, _, inf, _ in res if inf.status.err)),
+    }
+    if args.no_solo:
+        return out
     # one at a time, FIFO in arrival order: each request alone on its own
     # context (upload, solve, download), queueing simulated from its duration
     progress(f"served: makespan {makespan:.2f}s; now one at a time")
+    builder = RequestBuilder(draws)
     identical = True
     finish, solo_tts, busy = 0.0, [], 0.0
-    for j, ((U, inf), (arr, p, U0, t_end, cap)) in enumerate(zip(res, reqs)):
-        if j % 8 == 0:
-            progress(f"served: one at a time {j}/{len(reqs)}")
+    for j, ((_, _, _, inf, dig), d) in enumerate(zip(res, draws)):
+        if j % 16 == 0:
+            progress(f"served: one at a time {j}/{len(draws)}")
+        p, U0, t_end, cap = builder.get(j)
         t0 = time.perf_counter()
         with H.Context(p) as ctx:
             ctx.set_state(U0)
@@ -351,30 +414,17 @@ def run_served(args):
             Us = ctx.get_state()
         dur = time.perf_counter() - t0
         busy += dur
-        identical &= (bool(np.array_equal(Us, U, equal_nan=True)) and st.steps == inf.status.steps
-                      and st.err == inf.status.err)
-        finish = max(finish, arr) + dur
-        solo_tts.append(finish - arr)
+        identical &= digest(Us) == dig and st.steps == inf.status.steps and st.err == inf.status.err
+        finish = max(finish, d[0]) + dur
+        solo_tts.append(finish - d[0])
+    builder.close()
     solo_tts = np.array(solo_tts)
-    value = cell_steps / makespan / 1e6
-    out = {
-        "metric": METRIC, "value": round(value, 3), "unit": "Mcell-updates/s", "n_gpus": 1,
-        "steps": int(sum(inf.status.steps for _, inf in res)), "warmup": 3, "ms_per_step": None,
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-        "config": {"workload": WL.served_name(args.rate, args.requests),
-                   "requests": args.requests, "rate_per_s": args.rate, "seed": args.seed,
-                   "policy": args.policy, "max_active_cells": args.active_cells,
-                   "cell_steps": int(cell_steps)},
-        "roofline": aggregate_roofline(variants, makespan),
-        "latency_s": {"p50": round(float(np.percentile(tts, 50)), 4), "p99": round(float(np.percentile(tts, 99)), 4),
-                      "makespan": round(makespan, 3)},
-        "one_at_a_time": {"value": round(cell_steps / busy / 1e6, 3), "unit": "Mcell-updates/s",
-                          "p50": round(float(np.percentile(solo_tts, 50)), 4),
-                          "p99": round(float(np.percentile(solo_tts, 99)), 4),
-                          "makespan": round(float(finish), 3)},
-        "identical_to_solo": identical,
-        "inadmissible_requests": int(sum(1 for _, inf in res if inf.status.err)),
-    }
+    out["one_at_a_time"] = {"value": round(cell_steps / busy / 1e6, 3), "unit": "Mcell-updates/s",
+                            "p50": round(float(np.percentile(solo_tts, 50)), 4),
+                            "p99": round(float(np.percentile(solo_tts, 99)), 4),
+                            "makespan": round(float(finish), 3)}
+    out["speedup_vs_one_at_a_time"] = round(busy / makespan, 3)
+    out["identical_to_solo"] = identical
     return out
 
 
@@ -527,6 +577,8 @@ def main():
     ap.add_argument("--active-cells", type=int, default=0,
                     help="served: cells advanced per round (0 = enough to fill the device, -1 = no cap)")
     ap.add_argument("--seed", type=int, default=2024)
+    ap.add_argument("--no-solo", action="store_true",
+                    help="batched and served: skip the one-at-a-time comparison")
     ap.add_argument("--no-fma", action="store_true",
                     help="single-grid configs: skip timing the FMA-contracted build (HYPE_OPT_FMA) beside the exact one")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",

@@ -65,6 +65,25 @@ int hyped_free(void* p) {
   return HYPE_OK;
 }
 
+// pinned host memory (staging of asynchronous copies)
+int hyped_host_alloc(void** p, size_t bytes) {
+  *p = nullptr;
+  if (bytes == 0) bytes = 8;
+  hipError_t e = hipHostMalloc(p, bytes, hipHostMallocDefault);
+  if (e == hipErrorOutOfMemory) {
+    (void)hipGetLastError();
+    snprintf(g_err, sizeof(g_err), "hipHostMalloc(%zu): out of memory", bytes);
+    return HYPE_E_NOMEM;
+  }
+  HCHECK(e, "hipHostMalloc");
+  return HYPE_OK;
+}
+
+int hyped_host_free(void* p) {
+  if (p) HCHECK(hipHostFree(p), "hipHostFree");
+  return HYPE_OK;
+}
+
 int hyped_memcpy_h2d(void* dst, const void* src, size_t bytes, void* s) {
   HCHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, (hipStream_t)s), "hipMemcpyAsync H2D");
   return HYPE_OK;
@@ -210,6 +229,11 @@ int hyped_event_destroy(void* ev) {
 
 int hyped_event_record(void* ev, void* s) {
   HCHECK(hipEventRecord((hipEvent_t)ev, (hipStream_t)s), "hipEventRecord");
+  return HYPE_OK;
+}
+
+int hyped_event_sync(void* ev) {
+  HCHECK(hipEventSynchronize((hipEvent_t)ev), "hipEventSynchronize");
   return HYPE_OK;
 }
 

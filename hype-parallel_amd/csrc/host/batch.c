@@ -133,13 +133,22 @@ int hype_batch_get_state(hype_batch *b, int i, double *U, int mem)
 int hype_batch_begin(hype_batch *b, const double *t0, const double *t_end, const int *max_steps,
                      unsigned flags)
 {
-    int *slots, i, rc;
+    int *slots, *clip, i, rc;
     if (!b || !t_end || !max_steps) return HYPE_E_INVALID;
     slots = (int *)malloc((size_t)b->n * sizeof(int));
-    if (!slots) return HYPE_E_NOMEM;
-    for (i = 0; i < b->n; ++i) slots[i] = i;
-    rc = eng_start(&b->E, slots, b->n, t0, t_end, max_steps, (flags & HYPE_TS_CLIP) ? 1 : 0, NULL, NULL);
+    clip = (int *)malloc((size_t)b->n * sizeof(int));
+    if (!slots || !clip) {
+        free(slots);
+        free(clip);
+        return HYPE_E_NOMEM;
+    }
+    for (i = 0; i < b->n; ++i) {
+        slots[i] = i;
+        clip[i] = (flags & HYPE_TS_CLIP) ? 1 : 0;
+    }
+    rc = eng_start(&b->E, slots, b->n, t0, t_end, max_steps, clip, NULL, NULL);
     free(slots);
+    free(clip);
     if (rc == HYPE_OK) b->began = 1;
     return rc;
 }

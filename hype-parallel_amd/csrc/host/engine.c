@@ -8,6 +8,7 @@
 
 int eng_init(hype_engine *E, int cap, int target_rows, unsigned flags)
 {
+    int p;
     memset(E, 0, sizeof(*E));
     if (cap < 1) return HYPE_E_INVALID;
     HYPE_TRY(hyped_init());
@@ -26,6 +27,10 @@ int eng_init(hype_engine *E, int cap, int target_rows, unsigned flags)
     if (!E->inst_h || !E->P || !E->used || !E->group_of || !E->nsy || !E->fresh || !E->run) return HYPE_E_NOMEM;
     HYPE_TRY(hyped_malloc((void **)&E->inst_d, (size_t)cap * sizeof(hype_inst)));
     HYPE_TRY(hyped_malloc((void **)&E->st_d, (size_t)cap * sizeof(hype_dstate)));
+    for (p = 0; p < 2; ++p) {
+        HYPE_TRY(hyped_host_alloc((void **)&E->inst_stage[p], (size_t)cap * sizeof(hype_inst)));
+        HYPE_TRY(hyped_host_alloc((void **)&E->st_stage[p], (size_t)cap * sizeof(hype_dstate)));
+    }
     HYPE_TRY(hyped_stream_create(&E->stream));
     HYPE_TRY(hyped_event_create_sync(&E->ev_fork));
     /* time states start zeroed: max_steps = 0 and no history pointer, so a
@@ -41,16 +46,24 @@ void eng_free(hype_engine *E)
     if (E->graph) hyped_graph_destroy(E->graph);
     for (g = 0; g < E->ng; ++g) {
         eng_group *G = &E->g[g];
+        int p;
+        for (p = 0; p < 2; ++p) {
+            hyped_host_free(G->stage_tiles[p]);
+            hyped_host_free(G->stage_prime[p]);
+        }
         hyped_free(G->tiles_d);
         hyped_free(G->ntiles_d);
         hyped_free(G->prime_d);
         hyped_free(G->nprime_d);
-        hyped_free(G->queue_d);
         if (G->ev_done) hyped_event_destroy(G->ev_done);
         if (G->stream) hyped_stream_destroy(G->stream);
     }
     hyped_free(E->inst_d);
     hyped_free(E->st_d);
+    for (g = 0; g < 2; ++g) {
+        hyped_host_free(E->inst_stage[g]);
+        hyped_host_free(E->st_stage[g]);
+    }
     if (E->ev_fork) hyped_event_destroy(E->ev_fork);
     if (E->stream) hyped_stream_destroy(E->stream);
     free(E->inst_h);
@@ -60,6 +73,7 @@ void eng_free(hype_engine *E)
     free(E->nsy);
     free(E->fresh);
     free(E->run);
+    free(E->tmp);
     memset(E, 0, sizeof(*E));
 }
 
@@ -86,17 +100,21 @@ static int find_group(hype_engine *E, const hype_variant *v)
             hyped_malloc((void **)&G->prime_d, (size_t)G->tile_cap * sizeof(hype_tile)) != HYPE_OK ||
             hyped_malloc((void **)&G->ntiles_d, 2 * sizeof(int)) != HYPE_OK ||
             hyped_malloc((void **)&G->nprime_d, 2 * sizeof(int)) != HYPE_OK ||
-            hyped_malloc((void **)&G->queue_d, 4 * sizeof(unsigned)) != HYPE_OK ||
-            hyped_fill_u64(G->queue_d, 0ull, 2, E->stream) != HYPE_OK ||
             hyped_fill_u64(G->ntiles_d, 0ull, 1, E->stream) != HYPE_OK ||
             hyped_fill_u64(G->nprime_d, 0ull, 1, E->stream) != HYPE_OK ||
-            hyped_stream_create(&G->stream) != HYPE_OK || hyped_event_create_sync(&G->ev_done) != HYPE_OK)
+            hyped_stream_create(&G->stream) != HYPE_OK || hyped_event_create_sync(&G->ev_done) != HYPE_OK ||
+            hyped_host_alloc((void **)&G->stage_tiles[0], (size_t)G->tile_cap * sizeof(hype_tile)) != HYPE_OK ||
+            hyped_host_alloc((void **)&G->stage_tiles[1], (size_t)G->tile_cap * sizeof(hype_tile)) != HYPE_OK ||
+            hyped_host_alloc((void **)&G->stage_prime[0], (size_t)G->prime_cap * sizeof(hype_tile)) != HYPE_OK ||
+            hyped_host_alloc((void **)&G->stage_prime[1], (size_t)G->prime_cap * sizeof(hype_tile)) != HYPE_OK)
             return -1;
         G->grid = E->ncu * hyped_blocks_per_cu(v);
         if (G->grid < 1) G->grid = E->ncu;
         G->bwaves = hyped_block_waves(v);
-        /* a new branch: the step graph must be captured again */
+        /* a new branch: the step graph must be captured again (once the
+         * replays already queued have run) */
         if (E->graph) {
+            if (hyped_stream_sync(E->stream) != HYPE_OK) return -1;
             hyped_graph_destroy(E->graph);
             E->graph = NULL;
         }
@@ -184,7 +202,7 @@ int eng_total_tiles(hype_engine *E) { return count_tiles(E, -1, 0); }
  * caller rewrites the whole table afterwards (eng_commit). */
 static int ensure_step_cap(hype_engine *E, eng_group *G, int n)
 {
-    int cap = G->tile_cap;
+    int cap = G->tile_cap, p;
     if (n <= cap) return HYPE_OK;
     while (cap < n) cap *= 2;
     HYPE_TRY(hyped_stream_sync(E->stream));
@@ -194,6 +212,11 @@ static int ensure_step_cap(hype_engine *E, eng_group *G, int n)
     G->ntiles = 0;
     HYPE_TRY(hyped_fill_u64(G->ntiles_d, 0ull, 1, E->stream));
     HYPE_TRY(hyped_malloc((void **)&G->tiles_d, (size_t)cap * sizeof(hype_tile)));
+    for (p = 0; p < 2; ++p) {
+        hyped_host_free(G->stage_tiles[p]);
+        G->stage_tiles[p] = NULL;
+        HYPE_TRY(hyped_host_alloc((void **)&G->stage_tiles[p], (size_t)cap * sizeof(hype_tile)));
+    }
     G->tile_cap = cap;
     if (E->graph) {
         hyped_graph_destroy(E->graph);
@@ -206,7 +229,7 @@ static int ensure_step_cap(hype_engine *E, eng_group *G, int n)
  * and the stepping table and its length are left as they are */
 static int ensure_prime_cap(hype_engine *E, eng_group *G, int n)
 {
-    int cap = G->prime_cap;
+    int cap = G->prime_cap, p;
     if (n <= cap) return HYPE_OK;
     while (cap < n) cap *= 2;
     HYPE_TRY(hyped_stream_sync(E->stream));
@@ -214,6 +237,11 @@ static int ensure_prime_cap(hype_engine *E, eng_group *G, int n)
     G->prime_d = NULL;
     G->prime_cap = 0;
     HYPE_TRY(hyped_malloc((void **)&G->prime_d, (size_t)cap * sizeof(hype_tile)));
+    for (p = 0; p < 2; ++p) {
+        hyped_host_free(G->stage_prime[p]);
+        G->stage_prime[p] = NULL;
+        HYPE_TRY(hyped_host_alloc((void **)&G->stage_prime[p], (size_t)cap * sizeof(hype_tile)));
+    }
     G->prime_cap = cap;
     return HYPE_OK;
 }
@@ -261,43 +289,46 @@ int eng_set_rows(hype_engine *E, int rows)
     return HYPE_OK;
 }
 
-static int cmp_tile_h(const void *a, const void *b)
-{
-    const hype_tile *x = (const hype_tile *)a, *y = (const hype_tile *)b;
-    if (x->h != y->h) return y->h - x->h; /* taller first: longest-job-first queue */
-    if (x->inst != y->inst) return x->inst - y->inst;
-    if (x->y0 != y->y0) return x->y0 - y->y0;
-    return x->x0 - y->x0;
-}
-
+/* Tiles of the group's slots (all running ones, or the fresh ones) in slot
+ * order, each instance row-major, then banded by XCD (hype_xcd_order): the
+ * device hands workgroup b to XCD b % 8, and each XCD sweeps a contiguous
+ * band, so the halo rows that neighbouring tiles share are read through one
+ * L2. */
 static int build_tiles(hype_engine *E, int g, int only_fresh, hype_tile *buf, int cap)
 {
     int s, n = 0;
+    if (E->tmp_cap < cap) {
+        hype_tile *t = (hype_tile *)realloc(E->tmp, (size_t)cap * sizeof(hype_tile));
+        if (!t) return -1;
+        E->tmp = t;
+        E->tmp_cap = cap;
+    }
     for (s = 0; s < E->cap; ++s) {
         int k, m;
         if (!E->used[s] || E->group_of[s] != g) continue;
         if (only_fresh ? !E->fresh[s] : !E->run[s]) continue;
         E->nsy[s] = rows_for(E, s);
         E->inst_h[s].nsy = E->nsy[s];
-        m = hype_make_tiles(s, E->P[s].nx, E->P[s].ny, hype_strips(E->P[s].nx), E->nsy[s], buf + n, cap - n);
+        m = hype_make_tiles(s, E->P[s].nx, E->P[s].ny, hype_strips(E->P[s].nx), E->nsy[s], E->tmp + n, cap - n);
         if (m < 0) return -1;
-        for (k = 0; k < m; ++k) buf[n + k].flags = (k == 0) ? 1 : 0;
+        for (k = 0; k < m; ++k) E->tmp[n + k].flags = (k == 0) ? 1 : 0;
         n += m;
     }
-    qsort(buf, (size_t)n, sizeof(hype_tile), cmp_tile_h);
+    hype_xcd_order_into(E->tmp, buf, n, HYPE_NXCD);
     return n;
 }
 
+void eng_flip(hype_engine *E) { E->par ^= 1; }
+
 int eng_commit(hype_engine *E)
 {
-    int total = count_tiles(E, -1, 0);
-    hype_tile *buf = (hype_tile *)malloc((size_t)(total > 0 ? total : 1) * sizeof(hype_tile));
     int g, rc = HYPE_OK;
-    if (!buf) return HYPE_E_NOMEM;
     for (g = 0; g < E->ng && rc == HYPE_OK; ++g) {
         eng_group *G = &E->g[g];
+        hype_tile *buf;
         int n;
         if ((rc = ensure_step_cap(E, G, count_tiles(E, g, 0))) != HYPE_OK) break;
+        buf = G->stage_tiles[E->par];
         n = build_tiles(E, g, 0, buf, G->tile_cap);
         if (n < 0) {
             rc = HYPE_E_INVALID; /* a grid the tiler cannot split */
@@ -307,13 +338,17 @@ int eng_commit(hype_engine *E)
         if (n > 0) rc = hyped_memcpy_h2d(G->tiles_d, buf, (size_t)n * sizeof(hype_tile), E->stream);
         if (rc == HYPE_OK) rc = hyped_fill_u64(G->ntiles_d, (unsigned long long)(unsigned)n, 1, E->stream);
     }
-    if (rc == HYPE_OK)
-        rc = hyped_memcpy_h2d(E->inst_d, E->inst_h, (size_t)E->cap * sizeof(hype_inst), E->stream);
-    if (rc == HYPE_OK) rc = hyped_stream_sync(E->stream);
-    free(buf);
+    if (rc == HYPE_OK) {
+        memcpy(E->inst_stage[E->par], E->inst_h, (size_t)E->cap * sizeof(hype_inst));
+        rc = hyped_memcpy_h2d(E->inst_d, E->inst_stage[E->par], (size_t)E->cap * sizeof(hype_inst), E->stream);
+    }
+    if (rc == HYPE_OK && !(E->flags & ENG_ASYNC)) rc = hyped_stream_sync(E->stream);
     return rc;
 }
 
+/* One workgroup per table entry: the launch covers the table's capacity and
+ * workgroups past the device-resident count return at once, so the tables
+ * may change between graph replays. */
 static hype_launch base(hype_engine *E, eng_group *G, int mode, int stage)
 {
     hype_launch L;
@@ -322,54 +357,48 @@ static hype_launch base(hype_engine *E, eng_group *G, int mode, int stage)
     L.insts = E->inst_d;
     L.mode = mode;
     L.stage = stage;
-    L.grid = G->grid;
+    L.grid = 0;
     if (mode == HYPE_MODE_PRIME) {
         L.tiles = G->prime_d;
         L.ntiles_dev = G->nprime_d;
-        L.queue = G->queue_d + 2;
+        L.ntiles = G->prime_cap;
     } else {
         L.tiles = G->tiles_d;
         L.ntiles_dev = G->ntiles_d;
-        L.queue = G->queue_d;
+        L.ntiles = G->tile_cap;
     }
-    L.ntiles = 1; /* launch even when the device-resident count is 0 */
     return L;
 }
 
 int eng_start(hype_engine *E, const int *slots, int n, const double *t0, const double *t_end,
-              const int *max_steps, int clip, double **hist, const int *hist_len)
+              const int *max_steps, const int *clip, double **hist, const int *hist_len)
 {
-    hype_tile *buf;
-    hype_dstate *d;
+    hype_dstate *d = E->st_stage[E->par];
     int k, g, m, rc = HYPE_OK;
     for (k = 0; k < n; ++k)
         if (slots[k] < 0 || slots[k] >= E->cap || !E->used[slots[k]]) return HYPE_E_INVALID;
-    /* the new time states: one host array, copied slot by slot and waited
-     * for before the array is freed */
-    d = (hype_dstate *)calloc((size_t)(n > 0 ? n : 1), sizeof(hype_dstate));
-    if (!d) return HYPE_E_NOMEM;
+    if (n > E->cap) return HYPE_E_INVALID;
+    /* the new time states, staged in pinned memory */
+    memset(d, 0, (size_t)n * sizeof(hype_dstate));
     for (k = 0; k < n && rc == HYPE_OK; ++k) {
         const int s = slots[k];
         d[k].t = t0 ? t0[k] : 0.0;
         d[k].t_end = t_end[k];
         d[k].err[0] = d[k].err[1] = d[k].err_rhs = ~0ull;
         d[k].max_steps = max_steps[k];
-        d[k].clip = clip;
+        d[k].clip = clip ? (clip[k] != 0) : 0;
         d[k].dt_hist = hist ? hist[k] : NULL;
         d[k].hist_len = (hist && hist_len) ? hist_len[k] : 0;
         rc = hyped_memcpy_h2d(E->st_d + s, &d[k], sizeof(hype_dstate), E->stream);
         E->fresh[s] = 2; /* marked for priming */
     }
-    if (rc == HYPE_OK) rc = hyped_stream_sync(E->stream);
-    free(d);
     if (rc != HYPE_OK) return rc;
     for (k = 0; k < E->cap; ++k) E->fresh[k] = (E->fresh[k] == 2);
-    m = count_tiles(E, -1, 1);
-    buf = (hype_tile *)malloc((size_t)(m > 0 ? m : 1) * sizeof(hype_tile));
-    if (!buf) return HYPE_E_NOMEM;
     for (g = 0; g < E->ng && rc == HYPE_OK; ++g) {
         eng_group *G = &E->g[g];
+        hype_tile *buf;
         if ((rc = ensure_prime_cap(E, G, count_tiles(E, g, 1))) != HYPE_OK) break;
+        buf = G->stage_prime[E->par];
         m = build_tiles(E, g, 1, buf, G->prime_cap);
         if (m < 0) {
             rc = HYPE_E_UNSUPPORTED;
@@ -385,9 +414,7 @@ int eng_start(hype_engine *E, const int *slots, int n, const double *t0, const d
         }
     }
     for (k = 0; k < E->cap; ++k) E->fresh[k] = 0;
-    /* the tile buffer is pageable host memory: wait for its copies */
-    if (rc == HYPE_OK) rc = hyped_stream_sync(E->stream);
-    free(buf);
+    if (rc == HYPE_OK && !(E->flags & ENG_ASYNC)) rc = hyped_stream_sync(E->stream);
     return rc;
 }
 
@@ -395,6 +422,8 @@ static int capture(hype_engine *E)
 {
     int g, s, rc, rc2;
     void *graph = NULL;
+    /* capture on an idle stream (a new variant or a grown table; rare) */
+    HYPE_TRY(hyped_stream_sync(E->stream));
     HYPE_TRY(hyped_graph_begin(E->stream));
     rc = hyped_event_record(E->ev_fork, E->stream);
     for (g = 0; g < E->ng && rc == HYPE_OK; ++g) {

@@ -3,9 +3,10 @@
  *
  * Instances live in slots of a fixed-capacity table on the device.  Every
  * kernel variant in use (system x path x order) has a group: a tile table on
- * the device whose length is also read from device memory, a persistent-
- * queue counter and a branch stream.  One time step of all instances is one
- * persistent launch per RK stage per group; the launches of all groups are
+ * the device whose length is also read from device memory, and a branch
+ * stream.  One time step of all instances is one launch per RK stage per
+ * group over the table's capacity (workgroups past the length return at
+ * once); the launches of all groups are
  * captured once into a hipGraph with one parallel branch per group, and
  * replayed.  Admitting or retiring instances rewrites the tables between
  * replays, never the graph.
@@ -17,6 +18,10 @@
 
 #define HYPE_MAX_GROUPS 48  /* system x path x order 1..3 x contraction, plus inlet builds */
 
+/* eng_init flags */
+#define ENG_ASYNC 0x1u      /* table and time-state uploads do not wait for the
+                               stream: the caller orders them (served rounds) */
+
 typedef struct {
     hype_variant v;
     int nstages;
@@ -24,11 +29,12 @@ typedef struct {
     int *ntiles_d;          /* its device-resident length */
     hype_tile *prime_d;     /* priming table of newly started instances */
     int *nprime_d;
-    unsigned *queue_d;      /* {next, finished} for stepping, then priming */
     int tile_cap;           /* capacity of the stepping table */
     int prime_cap;          /* capacity of the priming table */
+    hype_tile *stage_tiles[2];  /* pinned staging of the two tables, per */
+    hype_tile *stage_prime[2];  /* upload parity (capacities as above) */
     int ntiles, nprime;     /* host mirrors */
-    int grid;               /* persistent workgroups */
+    int grid;               /* resident workgroups of the variant's kernel */
     int bwaves;             /* waves per workgroup (2: tile halves on two waves) */
     void *stream;           /* graph branch */
     void *ev_done;
@@ -49,10 +55,20 @@ typedef struct {
     void *ev_fork;
     void *graph;            /* one captured step */
     int ncu;
-    unsigned flags;
+    unsigned flags;         /* ENG_* */
+    int par;                /* staging parity of the next uploads */
+    hype_inst *inst_stage[2];   /* pinned staging: descriptors, fresh time states */
+    hype_dstate *st_stage[2];
+    hype_tile *tmp;         /* host scratch of the tile builder */
+    int tmp_cap;
 } hype_engine;
 
 int eng_init(hype_engine *E, int cap, int target_rows, unsigned flags);
+/* Uploads go through pinned staging buffers of two parities.  An ENG_ASYNC
+ * engine's caller flips the parity (eng_flip) once per round and makes sure
+ * the uploads of the round before the last have completed before it stages
+ * new ones; other engines wait for every upload before returning. */
+void eng_flip(hype_engine *E);
 void eng_free(hype_engine *E);
 /* place an instance in a slot (its U, A, B device buffers) */
 int eng_attach(hype_engine *E, int slot, const hype_problem *p, double *U, double *A, double *B,
@@ -63,9 +79,10 @@ int eng_detach(hype_engine *E, int slot);
 int eng_set_running(hype_engine *E, int slot, int on);
 /* recompute the tile tables of every group from the running slots */
 int eng_commit(hype_engine *E);
-/* initialise the time state of the fresh slots and run their priming call */
+/* initialise the time state of the fresh slots and run their priming call
+ * (clip: per slot, may be NULL; at most one call per staging parity) */
 int eng_start(hype_engine *E, const int *slots, int n, const double *t0, const double *t_end,
-              const int *max_steps, int clip, double **hist, const int *hist_len);
+              const int *max_steps, const int *clip, double **hist, const int *hist_len);
 /* enqueue nsteps steps of every used slot (graph replays) */
 int eng_step(hype_engine *E, int nsteps);
 int eng_sync(hype_engine *E);

@@ -4,25 +4,33 @@
  * The reference solves one configured problem per program run
  * (hype.c:14-456).  The server accepts a stream of independent solve
  * requests, each with its own system, grid, end time and step budget, and
- * keeps as many of them resident in HBM as a byte budget allows.  Every
- * round (hype_server_poll) it
- *   1. reads the time state of the running instances (one small copy),
- *   2. retires the finished ones: result to the caller's buffer, slot and
- *      HBM block freed,
- *   3. admits queued requests while slots and the HBM budget allow, in
- *      arrival order or least work (cells x nvar x step budget) first,
- *   4. picks the instances advanced this round: resident ones in arrival
- *      order, or least remaining work first, up to a cap on their cells
- *      (the others keep their state in HBM and wait),
- *   5. rebuilds the tile tables and primes the new instances, and
- *   6. enqueues the next steps of the running set: one persistent launch
- *      per RK stage per kernel variant, replayed from a captured graph.
- * An instance's arithmetic does not depend on what else runs: results are
- * bitwise those of a solo hype_ctx solve.
+ * keeps as many of them resident in HBM as a byte budget allows.
  *
- * Device states live in one arena allocated at creation; blocks are placed
- * first-fit and coalesced on release, so admission never calls the device
- * allocator.
+ * Rounds.  Each hype_server_poll prepares and enqueues one round: a number of
+ * steps of the running instances (one persistent launch per RK stage per
+ * kernel variant, replayed from a captured graph), followed by a copy of
+ * every time state into pinned host memory.  Two rounds are kept in flight,
+ * so the device never waits for the host: a poll waits only for the round
+ * before the last one, and decides from that round's time states
+ *   1. which instances have finished: their results are copied to the
+ *      caller's buffers on a separate copy stream (a finished instance is
+ *      no longer read or written by the steps still queued), and their slot
+ *      and HBM block are freed;
+ *   2. which queued requests to admit, while slots and the HBM budget allow,
+ *      in arrival order or least work (cells x nvar x step budget) first;
+ *      their initial states go up on the copy stream;
+ *   3. which resident instances to advance: in arrival order, or least
+ *      remaining work first, up to a cap on their cells (the others keep
+ *      their state in HBM and wait).
+ * Tile tables, descriptors and the new time states are staged in pinned
+ * memory and copied on the compute stream, behind the round still running,
+ * so they never change under a running step; the priming launches of the new
+ * instances wait for their uploads.
+ *
+ * An instance's arithmetic does not depend on what else runs: results are
+ * bitwise those of a solo hype_ctx solve.  Device states live in one arena
+ * allocated at creation; blocks are placed first-fit and coalesced on
+ * release, so admission never calls the device allocator.
  */
 #define _GNU_SOURCE
 #include <stdlib.h>
@@ -64,16 +72,27 @@ struct hype_server {
     int nqueue;
     long long active_cells;
     int *slot_req;          /* request resident in each slot, or -1 */
-    int *is_new;            /* admitted this round (time state not read yet) */
+    long long *slot_round;  /* first round that steps the slot's request */
     int *order;             /* scratch for the running-set selection */
     double *key;
+    int *fresh;             /* slots admitted in this poll */
+    int *clip;
+    double *t0, *t_end;
+    int *maxs;
     int active;
-    int steps;              /* steps enqueued by the last round */
     double ms_per_step;     /* measured GPU time of one step of the mix */
-    void *ev0, *ev1;
-    int timing;             /* ev0/ev1 bracket the last round's steps */
-    hype_dstate *d;
-    struct timespec t0;
+    void *copy;             /* copy stream: initial states up, results down */
+    void *ev_up;            /* uploads of this poll done */
+    /* rounds: number enqueued so far, and per parity the pinned snapshot of
+     * the time states after the round, its events and its step count */
+    long long round;
+    long long consumed;     /* rounds whose snapshot has been read */
+    hype_dstate *snap[2];
+    void *ev_snap[2], *ev0[2], *ev1[2];
+    int steps[2];
+    hype_dstate *d;         /* time states of the last consumed round */
+    long long d_round;      /* that round's number (-1: none) */
+    struct timespec t0c;
 };
 
 /* ---- arena ------------------------------------------------------------ */
@@ -155,7 +174,7 @@ static double now_s(const hype_server *s)
 {
     struct timespec t;
     clock_gettime(CLOCK_MONOTONIC, &t);
-    return (double)(t.tv_sec - s->t0.tv_sec) + 1e-9 * (double)(t.tv_nsec - s->t0.tv_nsec);
+    return (double)(t.tv_sec - s->t0c.tv_sec) + 1e-9 * (double)(t.tv_nsec - s->t0c.tv_nsec);
 }
 
 static size_t state_bytes(const hype_problem *p)
@@ -167,17 +186,29 @@ void hype_server_destroy(hype_server *s)
 {
     int k;
     if (!s) return;
+    if (s->copy) hyped_stream_sync(s->copy);
     for (k = 0; k < s->nreq; ++k) free(s->req[k].inlet_h);
     eng_free(&s->E);
     arena_free_all(&s->A);
-    if (s->ev0) hyped_event_destroy(s->ev0);
-    if (s->ev1) hyped_event_destroy(s->ev1);
+    for (k = 0; k < 2; ++k) {
+        hyped_host_free(s->snap[k]);
+        if (s->ev_snap[k]) hyped_event_destroy(s->ev_snap[k]);
+        if (s->ev0[k]) hyped_event_destroy(s->ev0[k]);
+        if (s->ev1[k]) hyped_event_destroy(s->ev1[k]);
+    }
+    if (s->ev_up) hyped_event_destroy(s->ev_up);
+    if (s->copy) hyped_stream_destroy(s->copy);
     free(s->req);
     free(s->queue);
     free(s->slot_req);
-    free(s->is_new);
+    free(s->slot_round);
     free(s->order);
     free(s->key);
+    free(s->fresh);
+    free(s->clip);
+    free(s->t0);
+    free(s->t_end);
+    free(s->maxs);
     free(s->d);
     free(s);
 }
@@ -186,7 +217,7 @@ int hype_server_create(const hype_server_opts *opts, hype_server **out)
 {
     hype_server *s;
     size_t budget;
-    int k, rc;
+    int k, rc, cap;
     if (!out) return HYPE_E_INVALID;
     *out = NULL;
     s = (hype_server *)calloc(1, sizeof(*s));
@@ -198,7 +229,7 @@ int hype_server_create(const hype_server_opts *opts, hype_server **out)
         free(s);
         return HYPE_E_INVALID;
     }
-    clock_gettime(CLOCK_MONOTONIC, &s->t0);
+    clock_gettime(CLOCK_MONOTONIC, &s->t0c);
     budget = s->o.hbm_budget_bytes;
     if ((rc = hyped_init()) != HYPE_OK) {
         free(s);
@@ -219,19 +250,37 @@ int hype_server_create(const hype_server_opts *opts, hype_server **out)
         int ncu = hyped_num_cus();
         s->o.max_active_cells = (long long)(ncu > 0 ? ncu : 256) * 4 * HYPE_TILE_W * 256;
     }
-    s->slot_req = (int *)malloc((size_t)s->o.max_active * sizeof(int));
-    s->is_new = (int *)calloc((size_t)s->o.max_active, sizeof(int));
-    s->order = (int *)malloc((size_t)s->o.max_active * sizeof(int));
-    s->key = (double *)malloc((size_t)s->o.max_active * sizeof(double));
-    s->d = (hype_dstate *)calloc((size_t)s->o.max_active, sizeof(hype_dstate));
-    if (!s->slot_req || !s->is_new || !s->order || !s->key || !s->d) {
+    cap = s->o.max_active;
+    s->slot_req = (int *)malloc((size_t)cap * sizeof(int));
+    s->slot_round = (long long *)calloc((size_t)cap, sizeof(long long));
+    s->order = (int *)malloc((size_t)cap * sizeof(int));
+    s->key = (double *)malloc((size_t)cap * sizeof(double));
+    s->fresh = (int *)malloc((size_t)cap * sizeof(int));
+    s->clip = (int *)malloc((size_t)cap * sizeof(int));
+    s->t0 = (double *)malloc((size_t)cap * sizeof(double));
+    s->t_end = (double *)malloc((size_t)cap * sizeof(double));
+    s->maxs = (int *)malloc((size_t)cap * sizeof(int));
+    s->d = (hype_dstate *)calloc((size_t)cap, sizeof(hype_dstate));
+    if (!s->slot_req || !s->slot_round || !s->order || !s->key || !s->fresh || !s->clip || !s->t0 ||
+        !s->t_end || !s->maxs || !s->d) {
         hype_server_destroy(s);
         return HYPE_E_NOMEM;
     }
-    for (k = 0; k < s->o.max_active; ++k) s->slot_req[k] = -1;
-    if ((rc = eng_init(&s->E, s->o.max_active, s->o.tile_rows, 0)) != HYPE_OK ||
-        (rc = arena_init(&s->A, budget)) != HYPE_OK || (rc = hyped_event_create(&s->ev0)) != HYPE_OK ||
-        (rc = hyped_event_create(&s->ev1)) != HYPE_OK) {
+    for (k = 0; k < cap; ++k) s->slot_req[k] = -1;
+    s->d_round = -1;
+    if ((rc = eng_init(&s->E, cap, s->o.tile_rows, ENG_ASYNC)) != HYPE_OK ||
+        (rc = arena_init(&s->A, budget)) != HYPE_OK || (rc = hyped_stream_create(&s->copy)) != HYPE_OK ||
+        (rc = hyped_event_create_sync(&s->ev_up)) != HYPE_OK) {
+        hype_server_destroy(s);
+        return rc;
+    }
+    for (k = 0; k < 2 && rc == HYPE_OK; ++k) {
+        if ((rc = hyped_host_alloc((void **)&s->snap[k], (size_t)cap * sizeof(hype_dstate))) != HYPE_OK ||
+            (rc = hyped_event_create_sync(&s->ev_snap[k])) != HYPE_OK ||
+            (rc = hyped_event_create(&s->ev0[k])) != HYPE_OK || (rc = hyped_event_create(&s->ev1[k])) != HYPE_OK)
+            break;
+    }
+    if (rc != HYPE_OK) {
         hype_server_destroy(s);
         return rc;
     }
@@ -294,12 +343,18 @@ static int pick(const hype_server *s)
     return best;
 }
 
+/* the last consumed time state describes the slot's current request */
+static int state_valid(const hype_server *s, int slot)
+{
+    return s->d_round >= 0 && s->d_round >= s->slot_round[slot];
+}
+
 static double remaining_work(const hype_server *s, int slot)
 {
     const srv_req *q = &s->req[s->slot_req[slot]];
     const hype_dstate *d = &s->d[slot];
     double steps = q->r.max_steps;
-    if (!s->is_new[slot]) {
+    if (state_valid(s, slot)) {
         steps = (double)(d->max_steps - d->step);
         if (d->dt > 0.0 && d->t_end < 1e300) {
             const double by_time = (d->t_end - d->t) / d->dt + 1.0;
@@ -350,22 +405,28 @@ static int finished(const hype_dstate *d, unsigned long long *err)
     return d->done || d->step >= d->max_steps || !(d->t < d->t_end) || e != ~0ull;
 }
 
-static int retire(hype_server *s, int slot, double t)
+/* Retire the finished request of a slot.  Its state is no longer touched by
+ * the steps still queued (they return at once for a finished instance), so
+ * the result is copied on the copy stream while they run; the slot and the
+ * HBM block are free once the copy has landed: later writes to the slot's
+ * descriptor and time state are ordered behind the queued steps, and later
+ * uploads to the block behind this copy. */
+static int retire(hype_server *s, int slot)
 {
     srv_req *q = &s->req[s->slot_req[slot]];
     const hype_dstate *d = &s->d[slot];
     unsigned long long e;
     finished(d, &e);
     if (q->r.U_out) {
-        HYPE_TRY(hyped_memcpy_d2h(q->r.U_out, s->A.base + q->off, state_bytes(&q->r.problem), s->E.stream));
-        HYPE_TRY(hyped_stream_sync(s->E.stream));
+        HYPE_TRY(hyped_memcpy_d2h(q->r.U_out, s->A.base + q->off, state_bytes(&q->r.problem), s->copy));
+        HYPE_TRY(hyped_stream_sync(s->copy));
     }
     q->st.t = d->t;
     q->st.dt = d->dt;
     q->st.steps = d->step;
     q->st.done = 1;
     q->st.err = e != ~0ull ? HYPE_E_INADMISSIBLE : HYPE_OK;
-    q->t_done = t;
+    q->t_done = now_s(s);
     q->state = HYPE_REQ_DONE;
     q->slot = -1;
     HYPE_TRY(eng_detach(&s->E, slot));
@@ -378,67 +439,43 @@ static int retire(hype_server *s, int slot, double t)
     return HYPE_OK;
 }
 
-static int start_fresh(hype_server *s, const int *slots, int n, int clip)
+/* read the time states after round k (waits for it) */
+static int consume(hype_server *s, long long k)
 {
-    double *t0, *t_end;
-    int *maxs, k, m = 0, rc;
-    if (n == 0) return HYPE_OK;
-    t0 = (double *)malloc((size_t)n * sizeof(double));
-    t_end = (double *)malloc((size_t)n * sizeof(double));
-    maxs = (int *)malloc((size_t)n * sizeof(int));
-    int *sl = (int *)malloc((size_t)n * sizeof(int));
-    if (!t0 || !t_end || !maxs || !sl) {
-        free(t0);
-        free(t_end);
-        free(maxs);
-        free(sl);
-        return HYPE_E_NOMEM;
-    }
-    for (k = 0; k < n; ++k) {
-        const srv_req *q = &s->req[s->slot_req[slots[k]]];
-        if (((q->r.flags & HYPE_TS_CLIP) != 0) != (clip != 0)) continue;
-        sl[m] = slots[k];
-        t0[m] = q->r.t0;
-        t_end[m] = q->r.t_end;
-        maxs[m] = q->r.max_steps;
-        ++m;
-    }
-    rc = m ? eng_start(&s->E, sl, m, t0, t_end, maxs, clip, NULL, NULL) : HYPE_OK;
-    free(t0);
-    free(t_end);
-    free(maxs);
-    free(sl);
-    return rc;
+    const int p = (int)(k & 1);
+    float ms = 0.0f;
+    HYPE_TRY(hyped_event_sync(s->ev_snap[p]));
+    memcpy(s->d, s->snap[p], (size_t)s->o.max_active * sizeof(hype_dstate));
+    s->d_round = k;
+    s->consumed = k + 1;
+    if (s->steps[p] > 0 && hyped_event_elapsed_ms(s->ev0[p], s->ev1[p], &ms) == HYPE_OK && ms > 0.0f)
+        s->ms_per_step = (double)ms / s->steps[p];
+    return HYPE_OK;
 }
 
 int hype_server_poll(hype_server *s, int *active, int *queued)
 {
-    int slot, changed = 0, nfresh = 0, rc = HYPE_OK;
-    int *fresh;
+    int slot, changed = 0, nfresh = 0, rc = HYPE_OK, k;
     double t;
     if (!s) return HYPE_E_INVALID;
-    fresh = (int *)malloc((size_t)s->o.max_active * sizeof(int));
-    if (!fresh) return HYPE_E_NOMEM;
-    /* 1-2: retire what finished during the last round's steps */
-    if (s->active > 0) {
-        if ((rc = eng_read_states(&s->E, s->d)) != HYPE_OK) goto out;
-        if (s->timing && s->steps > 0) {
-            float ms = 0.0f;
-            if (hyped_event_elapsed_ms(s->ev0, s->ev1, &ms) == HYPE_OK && ms > 0.0f)
-                s->ms_per_step = (double)ms / s->steps;
-        }
-        t = now_s(s);
+    /* the round before the last one; the device keeps the last.  With no
+     * instance left, the last one too, so that nothing stays unread. */
+    if (s->round - s->consumed >= 2 || (s->active == 0 && s->round > s->consumed))
+        if ((rc = consume(s, s->active == 0 ? s->round - 1 : s->round - 2)) != HYPE_OK) goto out;
+    /* 1: retire what had finished by then */
+    if (s->active > 0 && s->d_round >= 0) {
         for (slot = 0; slot < s->o.max_active; ++slot) {
             unsigned long long e;
-            if (s->slot_req[slot] < 0 || !finished(&s->d[slot], &e)) continue;
-            if ((rc = retire(s, slot, t)) != HYPE_OK) goto out;
+            if (s->slot_req[slot] < 0 || !state_valid(s, slot) || !finished(&s->d[slot], &e)) continue;
+            if ((rc = retire(s, slot)) != HYPE_OK) goto out;
             changed = 1;
         }
     }
-    /* 3: admit while a slot and HBM are free */
+    /* 2: admit while a slot and HBM are free; initial states go up on the
+     * copy stream while the queued steps run */
     t = now_s(s);
     while (s->nqueue > 0) {
-        int k = pick(s), id = s->queue[k];
+        int kq = pick(s), id = s->queue[kq];
         srv_req *q = &s->req[id];
         long long cells = (long long)q->r.problem.nx * q->r.problem.ny;
         double *U;
@@ -461,9 +498,9 @@ int hype_server_poll(hype_server *s, int *active, int *queued)
                 }
                 hype_inlet_table(&q->r.problem, q->inlet_h);
                 inlet = U + 3 * m;
-                rc = hyped_memcpy_h2d(inlet, q->inlet_h, ni * sizeof(double), s->E.stream);
+                rc = hyped_memcpy_h2d(inlet, q->inlet_h, ni * sizeof(double), s->copy);
             }
-            if (rc != HYPE_OK || (rc = hyped_memcpy_h2d(U, q->r.U0, m * sizeof(double), s->E.stream)) != HYPE_OK ||
+            if (rc != HYPE_OK || (rc = hyped_memcpy_h2d(U, q->r.U0, m * sizeof(double), s->copy)) != HYPE_OK ||
                 (rc = eng_attach(&s->E, slot, &q->r.problem, U, U + m, U + 2 * m, inlet)) != HYPE_OK) {
                 arena_release(&s->A, q->off, q->bytes);
                 free(q->inlet_h);
@@ -475,42 +512,56 @@ int hype_server_poll(hype_server *s, int *active, int *queued)
         q->state = HYPE_REQ_RUNNING;
         q->t_start = t;
         s->slot_req[slot] = id;
-        s->is_new[slot] = 1;
-        fresh[nfresh++] = slot;
+        s->slot_round[slot] = s->round; /* primed and first stepped in the round enqueued below */
+        s->fresh[nfresh] = slot;
+        s->clip[nfresh] = (q->r.flags & HYPE_TS_CLIP) != 0;
+        s->t0[nfresh] = q->r.t0;
+        s->t_end[nfresh] = q->r.t_end;
+        s->maxs[nfresh] = q->r.max_steps;
+        ++nfresh;
         ++s->active;
         s->active_cells += cells;
-        memmove(s->queue + k, s->queue + k + 1, (size_t)(s->nqueue - k - 1) * sizeof(int));
+        memmove(s->queue + kq, s->queue + kq + 1, (size_t)(s->nqueue - kq - 1) * sizeof(int));
         --s->nqueue;
         changed = 1;
     }
-    /* 4: the instances advanced this round */
-    if (s->active > 0 && select_running(s)) changed = 1;
-    /* 5: new tile tables, priming of the admitted instances */
-    if (changed && s->active > 0) {
+    if (s->active == 0) goto out;
+    /* 3: the instances advanced this round */
+    if (select_running(s)) changed = 1;
+    /* 4: tables, descriptors and new time states, staged for this round and
+     * copied behind the queued steps; the priming waits for the uploads */
+    eng_flip(&s->E);
+    if (changed) {
         if (s->o.tile_rows < 1) eng_set_rows(&s->E, eng_pick_rows(&s->E));
         if ((rc = eng_commit(&s->E)) != HYPE_OK) goto out;
-        if ((rc = start_fresh(s, fresh, nfresh, 1)) != HYPE_OK) goto out;
-        if ((rc = start_fresh(s, fresh, nfresh, 0)) != HYPE_OK) goto out;
+        if (nfresh > 0) {
+            if ((rc = hyped_event_record(s->ev_up, s->copy)) != HYPE_OK) goto out;
+            if ((rc = hyped_stream_wait(s->E.stream, s->ev_up)) != HYPE_OK) goto out;
+            if ((rc = eng_start(&s->E, s->fresh, nfresh, s->t0, s->t_end, s->maxs, s->clip, NULL, NULL)) != HYPE_OK)
+                goto out;
+        }
     }
-    for (slot = 0; slot < nfresh; ++slot) s->is_new[fresh[slot]] = 0;
-    /* 6: the next steps of the running set */
-    s->steps = 0;
-    s->timing = 0;
-    if (s->active > 0) {
+    /* 5: the round's steps, then the snapshot of the time states */
+    {
+        const int p = (int)(s->round & 1);
         int n = s->o.steps_per_round;
         if (n < 1) {
             n = s->ms_per_step > 0.0 ? (int)(s->o.round_ms / s->ms_per_step) : 4;
             if (n < 1) n = 1;
             if (n > 256) n = 256;
         }
-        if ((rc = hyped_event_record(s->ev0, s->E.stream)) != HYPE_OK) goto out;
+        if ((rc = hyped_event_record(s->ev0[p], s->E.stream)) != HYPE_OK) goto out;
         if ((rc = eng_step(&s->E, n)) != HYPE_OK) goto out;
-        if ((rc = hyped_event_record(s->ev1, s->E.stream)) != HYPE_OK) goto out;
-        s->steps = n;
-        s->timing = 1;
+        if ((rc = hyped_event_record(s->ev1[p], s->E.stream)) != HYPE_OK) goto out;
+        if ((rc = hyped_memcpy_d2h(s->snap[p], s->E.st_d, (size_t)s->o.max_active * sizeof(hype_dstate),
+                                   s->E.stream)) != HYPE_OK)
+            goto out;
+        if ((rc = hyped_event_record(s->ev_snap[p], s->E.stream)) != HYPE_OK) goto out;
+        s->steps[p] = n;
+        ++s->round;
     }
 out:
-    free(fresh);
+    for (k = 0; k < nfresh; ++k) s->fresh[k] = -1;
     if (active) *active = s->active;
     if (queued) *queued = s->nqueue;
     return rc;

@@ -126,6 +126,9 @@ int hyped_stage_table(int order, int s, int *ibuf, double *coef);
 int hyped_init(void);
 int hyped_malloc(void **p, size_t bytes);
 int hyped_free(void *p);
+/* pinned host memory, for copies that do not block the host */
+int hyped_host_alloc(void **p, size_t bytes);
+int hyped_host_free(void *p);
 int hyped_memcpy_h2d(void *dst, const void *src, size_t bytes, void *stream);
 int hyped_memcpy_d2h(void *dst, const void *src, size_t bytes, void *stream);
 int hyped_memcpy_d2d(void *dst, const void *src, size_t bytes, void *stream);
@@ -157,6 +160,7 @@ int hyped_event_create(void **ev);
 int hyped_event_destroy(void *ev);
 int hyped_event_record(void *ev, void *stream);
 int hyped_event_elapsed_ms(void *a, void *b, float *ms);
+int hyped_event_sync(void *ev);
 int hyped_event_create_sync(void **ev);   /* no timing: fork/join in graphs */
 int hyped_stream_wait(void *stream, void *ev);
 

@@ -367,8 +367,10 @@ int hype_server_create(const hype_server_opts *opts, hype_server **out);
 void hype_server_destroy(hype_server *s);
 /* queue a request (HYPE_E_NOMEM if it can never fit the budget) */
 int hype_server_submit(hype_server *s, const hype_request *r, int *id);
-/* one round: retire, admit, enqueue steps; returns the running and queued
- * counts.  Blocks until the previous round's steps have finished. */
+/* one round: retire, admit, enqueue steps; returns the resident and queued
+ * counts.  Two rounds stay in flight: a poll waits only for the round before
+ * the last one, whose time states it acts on, while the device runs the
+ * last one. */
 int hype_server_poll(hype_server *s, int *active, int *queued);
 /* poll until every submitted request is done */
 int hype_server_drain(hype_server *s);
