@@ -230,6 +230,16 @@ def run_batched(args, with_cpu=True):
     cells = sum(p.nx * p.ny for p, _ in insts)
     progress(f"batched: {count} instances, {cells} cells, {steps} steps")
     with H.Batch([p for p, _ in insts]) as b:
+        # tile shape and layout measured on the live states around the
+        # tile-shape model's height (untimed; the states are set again below)
+        model_rows = b.tiling()[0]
+        cands = sorted({max(1, model_rows * f // 4) for f in (2, 3, 4, 5, 6)})
+        for i, (_, U0) in enumerate(insts):
+            b.set_state(i, U0)
+        tuned_rows, tuned_ms = b.autotune(cands, steps=4)
+        tuned = {"candidates": cands, "model_rows": model_rows, "rows": tuned_rows, "layout": b.layout(),
+                 "ms_per_step": round(tuned_ms / 4, 4)}
+        progress(f"batched: autotune {tuned}")
         for i, (_, U0) in enumerate(insts):
             b.set_state(i, U0)
         b.begin(max_steps=min(args.warmup, steps) or 1)
@@ -269,7 +279,7 @@ def run_batched(args, with_cpu=True):
         "config": {"workload": wname, "instances": count, "cells": cells, "nvar": 6, "order": 2,
                    "path": "RHSFunctionPrim", "scheme": "WENO3+Rusanov+SSPRK(4,3)", "cfl": 0.5,
                    "seed": args.seed, "tile_rows": rows, "tiles": ntiles, "variants": groups,
-                   "graph": True, "kernel_build": "exact"},
+                   "autotune": tuned, "graph": True, "kernel_build": "exact"},
         "roofline": roof,
         "valu_roofline": valu_roofline(wname, sec / (4 * steps)),
     }
@@ -305,6 +315,11 @@ class RequestBuilder:
             self.fut[self.next] = self.pool.submit(WL.served_request, system, n, steps, seed)
             self.next += 1
 
+    def ready(self, k):
+        """Whether request k has been built (schedules the next ones)."""
+        self._fill(k)
+        return self.fut[k].done()
+
     def get(self, k):
         """(problem, U0, t_end, cap) of request k (blocks until built)."""
         self._fill(k)
@@ -324,7 +339,17 @@ def serve_trace(draws, policy=H.SERVE_FIFO, active_cells=0):
     """Replay a stream through one hype_server in real time.  Returns per
     request (problem, t_end, cap, info, result digest) in submission order,
     and the makespan; results are hashed and released as they complete."""
-    builder = RequestBuilder(draws)
+    # the states of a stream of up to 1024 requests are built before the
+    # clock starts; a longer stream builds ahead of its arrivals, and the
+    # loop never waits for a build (it keeps polling, so the device never
+    # runs dry while the host computes initial states)
+    prebuild = len(draws) <= 1024
+    builder = RequestBuilder(draws, ahead=len(draws) if prebuild else 256,
+                             workers=min(16, os.cpu_count() or 1))
+    if prebuild:
+        builder.ready(0)
+        for f in builder.fut.values():
+            f.result()
     out = [None] * len(draws)
     with H.Server(policy=policy, max_active_cells=active_cells) as srv:
         # warm-up: a few tiny requests (kernel loading, graph capture)
@@ -342,7 +367,7 @@ def serve_trace(draws, policy=H.SERVE_FIFO, active_cells=0):
             if now - last > 10.0:
                 last = now
                 progress(f"served: t={now:.1f}s submitted={k} running/queued={len(pending)}")
-            while k < len(draws) and draws[k][0] <= now:
+            while k < len(draws) and draws[k][0] <= now and builder.ready(k):
                 p, U0, t_end, cap = builder.get(k)
                 pending[srv.submit(p, U0, t_end=t_end, max_steps=cap)] = (k, p, t_end, cap)
                 k += 1

@@ -470,20 +470,23 @@ struct Tile {
       }
       if (mode == HYPE_MODE_RHS) {
 #pragma unroll
-        for (int c = 0; c < NV; ++c) Out[idx + c] = rhs[c];
+        for (int c = 0; c < NV; ++c) __builtin_nontemporal_store(rhs[c], &Out[idx + c]);
       } else {
         // every operand of the record is loaded before the first store.  The
         // compiler cannot prove the output distinct from the inputs (it is
         // not, for the RK stages that update U in place), so loads
         // interleaved with the stores would each wait for memory: six round
-        // trips per row instead of one
+        // trips per row instead of one.  The stores are non-temporal: the
+        // output is read again only by the next launch, long after it has
+        // left L2, and streaming it past L2 keeps the window rows there for
+        // their re-read by this update and by the neighbouring tiles
         const auto out = Out;
         if (sd.op == 0) {
           double u[NV];
 #pragma unroll
           for (int c = 0; c < NV; ++c) u[c] = Uin[idx + c];
 #pragma unroll
-          for (int c = 0; c < NV; ++c) out[idx + c] = u[c] + cdt * rhs[c];
+          for (int c = 0; c < NV; ++c) __builtin_nontemporal_store(u[c] + cdt * rhs[c], &out[idx + c]);
         } else if (sd.op == 2) {
           // RK104 after its fifth stage: w0 += dt/6 F (VecAXPY); w1 =
           // 1/25 U + 9/25 w0 (VecAXPBYPCZ, c = 0) into U; w0 = 15 w1 - 5 w0
@@ -499,8 +502,8 @@ struct Tile {
           for (int c = 0; c < NV; ++c) {
             const double w0 = u[c] + cdt * rhs[c];
             const double w1 = sd.a * o[c] + sd.g * w0;
-            out[idx + c] = w1;
-            A[idx + c] = 15.0 * w1 + -5.0 * w0;
+            __builtin_nontemporal_store(w1, &out[idx + c]);
+            __builtin_nontemporal_store(15.0 * w1 + -5.0 * w0, &A[idx + c]);
           }
         } else {
           const auto X = Xb;
@@ -512,7 +515,7 @@ struct Tile {
             z[c] = Z[idx + c];
           }
 #pragma unroll
-          for (int c = 0; c < NV; ++c) out[idx + c] = sd.a * x[c] + cdt * rhs[c] + sd.g * z[c];
+          for (int c = 0; c < NV; ++c) __builtin_nontemporal_store(sd.a * x[c] + cdt * rhs[c] + sd.g * z[c], &out[idx + c]);
         }
       }
     };

@@ -224,17 +224,23 @@ int hype_batch_set_rows(hype_batch *b, int rows)
     return eng_commit(&b->E);
 }
 
-/* Measured tile-height tuning: time `steps` steps for every candidate height
- * (from the current states, which are advanced; re-set them afterwards) and
- * keep the fastest. */
+/* Measured tuning of the tile shape and LDS footprint: time `steps` steps
+ * of every candidate tile height, and, where an instance's system is built
+ * for both tile layouts (one wave per tile with one LDS window, or tiles
+ * split over two waves with two windows and the exchange areas), of both
+ * layouts; keep the fastest pair.  The states are advanced from the current
+ * ones: set them again afterwards. */
 int hype_batch_autotune(hype_batch *b, const int *cand, int ncand, int steps, int *best_rows,
                         double *best_ms)
 {
-    int k, i, best = -1;
+    int k, i, l, best = -1, best_layout = -1, nlay = 1, rc = HYPE_OK;
+    int layouts[2] = {-1, 1};
     double bms = 1e300;
     double *t_end;
     int *maxs;
     if (!b || !cand || ncand < 1 || steps < 1) return HYPE_E_INVALID;
+    for (i = 0; i < b->n; ++i)
+        if (hype_layouts(&b->E.P[i], hyped_pair_modes(b->E.P[i].system)) == 3) nlay = 2;
     t_end = (double *)malloc((size_t)b->n * sizeof(double));
     maxs = (int *)malloc((size_t)b->n * sizeof(int));
     if (!t_end || !maxs) {
@@ -246,27 +252,36 @@ int hype_batch_autotune(hype_batch *b, const int *cand, int ncand, int steps, in
         t_end[i] = 1e300;
         maxs[i] = 1 << 30;
     }
-    for (k = 0; k < ncand; ++k) {
-        double ms = 0.0;
-        int rc;
-        if (cand[k] < 1) continue;
-        if ((rc = hype_batch_set_rows(b, cand[k])) != HYPE_OK ||
-            (rc = hype_batch_begin(b, NULL, t_end, maxs, 0)) != HYPE_OK ||
-            (rc = hype_batch_advance(b, 1)) != HYPE_OK ||
-            (rc = hype_batch_advance_timed(b, steps, &ms)) != HYPE_OK) {
-            free(t_end);
-            free(maxs);
-            return rc;
-        }
-        if (ms < bms) {
-            bms = ms;
-            best = cand[k];
+    for (l = 0; l < nlay && rc == HYPE_OK; ++l) {
+        if ((rc = eng_set_layout(&b->E, layouts[l])) != HYPE_OK) break;
+        for (k = 0; k < ncand; ++k) {
+            double ms = 0.0;
+            if (cand[k] < 1) continue;
+            if ((rc = hype_batch_set_rows(b, cand[k])) != HYPE_OK ||
+                (rc = hype_batch_begin(b, NULL, t_end, maxs, 0)) != HYPE_OK ||
+                (rc = hype_batch_advance(b, 1)) != HYPE_OK ||
+                (rc = hype_batch_advance_timed(b, steps, &ms)) != HYPE_OK)
+                break;
+            if (ms < bms) {
+                bms = ms;
+                best = cand[k];
+                best_layout = layouts[l];
+            }
         }
     }
     free(t_end);
     free(maxs);
+    if (rc != HYPE_OK) return rc;
     if (best < 1) return HYPE_E_INVALID;
     if (best_rows) *best_rows = best;
     if (best_ms) *best_ms = bms;
+    HYPE_TRY(eng_set_layout(&b->E, best_layout));
     return hype_batch_set_rows(b, best);
+}
+
+int hype_batch_layout(hype_batch *b, int *layout)
+{
+    if (!b || !layout) return HYPE_E_INVALID;
+    *layout = b->E.layout;
+    return HYPE_OK;
 }

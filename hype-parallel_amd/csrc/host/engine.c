@@ -13,6 +13,7 @@ int eng_init(hype_engine *E, int cap, int target_rows, unsigned flags)
     if (cap < 1) return HYPE_E_INVALID;
     HYPE_TRY(hyped_init());
     E->cap = cap;
+    E->layout = -1;
     E->target_rows = target_rows;
     E->flags = flags;
     E->ncu = hyped_num_cus();
@@ -122,6 +123,25 @@ static int find_group(hype_engine *E, const hype_variant *v)
     }
 }
 
+/* the kernel variant of a problem in this engine */
+static int slot_variant(const hype_engine *E, const hype_problem *p, hype_variant *v)
+{
+    const int modes = hype_layouts(p, hyped_pair_modes(p->system));
+    if (!modes) return HYPE_E_UNSUPPORTED;
+    v->system = p->system;
+    v->prim = p->recons_prim;
+    v->order = p->order;
+    v->fma = (p->options & HYPE_OPT_FMA) != 0;
+    /* split tiles where the system has no other layout (or the problem asks
+     * for them), or where both are built and the engine prefers them; by
+     * default a batch keeps many waves per SIMD busy, where one-wave tiles
+     * hide latency better */
+    if (modes == 3 && E->layout >= 0) v->pair = E->layout;
+    else v->pair = (modes & 1) ? 0 : 1;
+    v->inlet = p->bc[0] == HYPE_BC_INLET;
+    return HYPE_OK;
+}
+
 int eng_attach(hype_engine *E, int slot, const hype_problem *p, double *U, double *A, double *B,
                const double *inlet)
 {
@@ -130,19 +150,7 @@ int eng_attach(hype_engine *E, int slot, const hype_problem *p, double *U, doubl
     if (slot < 0 || slot >= E->cap || E->used[slot]) return HYPE_E_INVALID;
     HYPE_TRY(hype_check_problem(p, &nv));
     if (p->order < 1) return HYPE_E_UNSUPPORTED; /* forward Euler stepping stays single-instance */
-    v.system = p->system;
-    v.prim = p->recons_prim;
-    v.order = p->order;
-    v.fma = (p->options & HYPE_OPT_FMA) != 0;
-    /* split tiles where the system has no other layout (or the problem asks
-     * for them); a batch keeps many waves per SIMD busy, where one-wave tiles
-     * hide latency better */
-    {
-        const int modes = hype_layouts(p, hyped_pair_modes(p->system));
-        if (!modes) return HYPE_E_UNSUPPORTED;
-        v.pair = (modes & 1) ? 0 : 1;
-    }
-    v.inlet = p->bc[0] == HYPE_BC_INLET;
+    HYPE_TRY(slot_variant(E, p, &v));
     g = find_group(E, &v);
     if (g < 0) return HYPE_E_DEVICE;
     E->P[slot] = *p;
@@ -166,6 +174,23 @@ int eng_detach(hype_engine *E, int slot)
     E->used[slot] = 0;
     E->run[slot] = 0;
     E->fresh[slot] = 0;
+    return HYPE_OK;
+}
+
+int eng_set_layout(hype_engine *E, int layout)
+{
+    int s;
+    if (layout < -1 || layout > 1) return HYPE_E_INVALID;
+    E->layout = layout;
+    for (s = 0; s < E->cap; ++s) {
+        hype_variant v;
+        int g;
+        if (!E->used[s]) continue;
+        HYPE_TRY(slot_variant(E, &E->P[s], &v));
+        g = find_group(E, &v);
+        if (g < 0) return HYPE_E_DEVICE;
+        E->group_of[s] = g;
+    }
     return HYPE_OK;
 }
 

@@ -49,6 +49,8 @@ typedef struct {
     int *used, *group_of, *nsy, *fresh;
     int *run;               /* used slots advanced by eng_step (default all) */
     int target_rows;        /* common tile height of all instances */
+    int layout;             /* tile layout where a system has both: -1 the
+                               default (one wave per tile), 0 one wave, 1 split */
     eng_group g[HYPE_MAX_GROUPS];
     int ng;
     void *stream;           /* main stream */
@@ -74,6 +76,10 @@ void eng_free(hype_engine *E);
 int eng_attach(hype_engine *E, int slot, const hype_problem *p, double *U, double *A, double *B,
                const double *inlet);
 int eng_detach(hype_engine *E, int slot);
+/* tile layout of the instances whose system is built for both (-1 default,
+ * 0 one wave per tile, 1 split tiles); moves them between groups, taking
+ * effect at the next eng_commit */
+int eng_set_layout(hype_engine *E, int layout);
 /* include (on = 1) or leave out a used slot from the steps; takes effect at
  * the next eng_commit.  A left-out instance keeps its state and time. */
 int eng_set_running(hype_engine *E, int slot, int on);

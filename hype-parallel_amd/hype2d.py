@@ -176,6 +176,7 @@ def lib():
     L.hype_batch_tiling.argtypes = [vp, ip, ip, ip]
     L.hype_batch_set_rows.argtypes = [vp, C.c_int]
     L.hype_batch_autotune.argtypes = [vp, ip, C.c_int, C.c_int, ip, dp]
+    L.hype_batch_layout.argtypes = [vp, ip]
     L.hype_server_create.argtypes = [P(ServerOpts), P(vp)]
     L.hype_server_destroy.argtypes = [vp]
     L.hype_server_destroy.restype = None
@@ -204,7 +205,7 @@ def exported_symbols():
         "hype_rankine_hugoniot", "hype_case_setup", "hype_batch_create", "hype_batch_destroy", "hype_batch_set_state",
         "hype_batch_get_state", "hype_batch_begin", "hype_batch_advance", "hype_batch_advance_timed",
         "hype_batch_sync", "hype_batch_last_error", "hype_batch_tiling", "hype_batch_set_rows",
-        "hype_batch_autotune", "hype_server_create", "hype_server_destroy", "hype_server_submit",
+        "hype_batch_autotune", "hype_batch_layout", "hype_server_create", "hype_server_destroy", "hype_server_submit",
         "hype_server_poll", "hype_server_drain", "hype_server_info", "hype_server_now",
         "hype_server_bytes_in_use", "hype_write_vec_binary", "hype_read_vec_binary", "hype_write_vts",
     ]
@@ -505,6 +506,13 @@ class Batch:
 
     def set_rows(self, rows):
         self._check(self.L.hype_batch_set_rows(self.h, rows), "hype_batch_set_rows")
+
+    def layout(self):
+        """Tile layout of the instances built for both (-1 default one wave
+        per tile, 1 split tiles)."""
+        v = C.c_int()
+        self._check(self.L.hype_batch_layout(self.h, C.byref(v)), "hype_batch_layout")
+        return v.value
 
     def autotune(self, candidates, steps=4):
         c = np.ascontiguousarray(candidates, dtype=np.int32)

@@ -307,10 +307,17 @@ int hype_batch_sync(hype_batch *b, hype_status *st);
 const char *hype_batch_last_error(hype_batch *b, double *x, double *y);
 int hype_batch_tiling(hype_batch *b, int *rows, int *tiles, int *groups);
 int hype_batch_set_rows(hype_batch *b, int rows);
-/* Time `steps` steps at every candidate tile height and keep the fastest.
- * Advances the current states: set them again afterwards. */
+/* Measured tuning of the tile shape and LDS footprint: times `steps` steps
+ * at every candidate tile height and, where an instance's system is built
+ * for both tile layouts (one wave per tile with one LDS window, or each
+ * tile split over a two-wave workgroup with two windows and exchange
+ * areas: Euler), in both layouts, and keeps the fastest pair.  Advances the
+ * current states: set them again afterwards. */
 int hype_batch_autotune(hype_batch *b, const int *cand, int ncand, int steps,
                         int *best_rows, double *best_ms);
+/* tile layout of the instances built for both: -1 the default (one wave
+ * per tile), 1 split tiles (as hype_batch_autotune left it) */
+int hype_batch_layout(hype_batch *b, int *layout);
 
 /* ======================================================================
  * Served mode: a stream of independent solve requests with continuous
