@@ -12,7 +12,8 @@ solved one at a time, and a CPU baseline (the FP64 C oracle with MPI ranks
 on the host cores, bounded sample).  The three fixed grids are timed in the
 same run and reported under "configs", each with its own roofline and CPU
 baseline: Kapila 1024x512 (500 steps), Euler 256^2 (200), Baer-Nunziato
-2048x1024 (300).
+2048x1024 (300); so is the served stream's first 64 requests (latency and
+throughput against one request at a time).
 
     python bench.py [--steps K] [--warmup W]            # batched + the three grids
     python bench.py --config kapila|euler|bn|batched     # one configuration
@@ -270,7 +271,7 @@ def run_batched(args, with_cpu=True):
     sec = ms / 1e3
     value = cells * steps / sec / 1e6
     roof = per_launch(aggregate_roofline({(6, 2): cells * steps}, sec, wname), 4 * steps)
-    roof["kernel"] = "stage_kernel (Kapila, prim, N=2, split tiles, XCD-banded tile table)"
+    roof["kernel"] = "stage_kernel (Kapila, prim, N=2, split tiles; persistent queue, tallest tiles first)"
     roof["avg_launch_us"] = round(sec / (4 * steps) * 1e6, 3)
     out = {
         "metric": METRIC, "value": round(value, 3), "unit": "Mcell-updates/s", "n_gpus": 1, "steps": steps,
@@ -530,7 +531,7 @@ def run_single(name, args, steps=None, fma=True, pcie=True, with_cpu=True):
     nv = H.NVAR[p.system]
     cells = p.nx * p.ny
     progress(f"{wname}: {steps} steps")
-    ms, st, U_end, (hrows, ntiles, grid), rhs_s = timed_run(p, U0, steps, args, pcie=pcie)
+    ms, st, U_end, (hrows, ntiles, grid), rhs_s = timed_run(p, U0, steps, args, pcie=pcie and not args.no_pcie)
     if st.steps != steps:
         raise SystemExit(f"{wname}: only {st.steps} of {steps} steps ran")
     fma_variant = None
@@ -581,6 +582,8 @@ def run_single(name, args, steps=None, fma=True, pcie=True, with_cpu=True):
 
 SUB_KEYS = ("value", "unit", "steps", "ms_per_step", "config", "roofline", "valu_roofline", "final_time",
             "pcie_inclusive", "fma_variant", "cpu_baseline")
+SERVED_KEYS = ("value", "unit", "config", "roofline", "latency_s", "one_at_a_time", "speedup_vs_one_at_a_time",
+               "identical_to_solo", "inadmissible_requests")
 
 
 def main():
@@ -602,6 +605,10 @@ def main():
     ap.add_argument("--active-cells", type=int, default=0,
                     help="served: cells advanced per round (0 = enough to fill the device, -1 = no cap)")
     ap.add_argument("--seed", type=int, default=2024)
+    ap.add_argument("--no-served", action="store_true",
+                    help="default run: skip the served stream's sub-record")
+    ap.add_argument("--no-pcie", action="store_true",
+                    help="single-grid configs: skip timing the RHS callback on host buffers")
     ap.add_argument("--no-solo", action="store_true",
                     help="batched and served: skip the one-at-a-time comparison")
     ap.add_argument("--no-fma", action="store_true",
@@ -628,6 +635,10 @@ def main():
         for name in ("kapila", "euler", "bn"):
             rec = run_single(name, args, fma=(name == "kapila"), pcie=(name == "kapila"))
             out["configs"][rec["config"]["workload"]] = {k: rec[k] for k in SUB_KEYS if k in rec}
+        # the served stream's first 64 requests, against one at a time
+        if not args.no_served:
+            rec = run_served(argparse.Namespace(**{**vars(args), "requests": 64}))
+            out["configs"][rec["config"]["workload"]] = {k: rec[k] for k in SERVED_KEYS if k in rec}
     else:
         out = run_single(args.config, args, steps=args.steps)
     print(json.dumps(out))

@@ -2,11 +2,11 @@
  * served drivers (host side, C).
  *
  * Instances live in slots of a fixed-capacity table on the device.  Every
- * kernel variant in use (system x path x order) has a group: a tile table on
- * the device whose length is also read from device memory, and a branch
- * stream.  One time step of all instances is one launch per RK stage per
- * group over the table's capacity (workgroups past the length return at
- * once); the launches of all groups are
+ * kernel variant in use (system x path x order x layout) has a group: a
+ * tile table on the device whose length is also read from device memory, a
+ * persistent-queue counter and a branch stream.  One time step of all
+ * instances is one persistent launch per RK stage per group, whose
+ * workgroups pull tiles tallest first; the launches of all groups are
  * captured once into a hipGraph with one parallel branch per group, and
  * replayed.  Admitting or retiring instances rewrites the tables between
  * replays, never the graph.
@@ -33,8 +33,10 @@ typedef struct {
     int prime_cap;          /* capacity of the priming table */
     hype_tile *stage_tiles[2];  /* pinned staging of the two tables, per */
     hype_tile *stage_prime[2];  /* upload parity (capacities as above) */
+    unsigned *queue_d;      /* persistent-queue counters {next, finished} of
+                               the stepping table, then of the priming table */
     int ntiles, nprime;     /* host mirrors */
-    int grid;               /* resident workgroups of the variant's kernel */
+    int grid;               /* persistent workgroups (all resident at once) */
     int bwaves;             /* waves per workgroup (2: tile halves on two waves) */
     void *stream;           /* graph branch */
     void *ev_done;
@@ -61,8 +63,6 @@ typedef struct {
     int par;                /* staging parity of the next uploads */
     hype_inst *inst_stage[2];   /* pinned staging: descriptors, fresh time states */
     hype_dstate *st_stage[2];
-    hype_tile *tmp;         /* host scratch of the tile builder */
-    int tmp_cap;
 } hype_engine;
 
 int eng_init(hype_engine *E, int cap, int target_rows, unsigned flags);
