@@ -201,7 +201,16 @@ struct Tile {
   static constexpr int WIN = NSLOT * NV * COLS;
   static constexpr int XCH = NGF * (NV + 1) * HYPE_WAVE;
   static constexpr int DEFER = 4 * NV * HYPE_WAVE;
-  static constexpr int LDS = WAVES * WIN + (PAIR ? 2 * XCH + DEFER : 0);
+  // then, for long records, per wave, the staging of one row of output
+  // records.  Each lane storing its own record writes NV doubles at a stride
+  // of NV doubles per instruction; with non-temporal stores, which are not
+  // merged in L2, 9-double records wrote 1.8x their bytes to HBM, 6-double
+  // records 1.2x.  Staging a row through LDS makes every store a whole line,
+  // at the price of two wave barriers per row: even for 9 doubles, and 2-3 %
+  // slower for 6
+  static constexpr bool STAGE_ROWS = NV > 6;
+  static constexpr int OST = STAGE_ROWS ? 2 * NV * HYPE_TILE_W : 0;  // Out and A
+  static constexpr int LDS = WAVES * WIN + (PAIR ? 2 * XCH + DEFER : 0) + WAVES * OST;
 
   // value of component c at node/face point p from its modal coefficients
   // (rhs_function_prim.c:382-388, 425-440)
@@ -315,6 +324,7 @@ struct Tile {
     double* defer = xch + 2 * XCH;
     auto X_ = [&](int side, int q, int c) -> double& { return xch[side * XCH + (q * (NV + 1) + c) * HYPE_WAVE + lane]; };
     auto D_ = [&](int k, int c) -> double& { return defer[(k * NV + c) * HYPE_WAVE + lane]; };
+    double* ost = lds + WAVES * WIN + (PAIR ? 2 * XCH + DEFER : 0) + (PAIR ? wave * OST : 0);
     const int rbase = r_first - 2;  // window row of slot 0
     auto slot = [&](int r) { return (r - rbase) % NSLOT; };
     auto W_ = [&](int s, int c, int k) -> double& { return win[(s * NV + c) * COLS + k]; };
@@ -454,12 +464,41 @@ struct Tile {
       }
     };
 
+    // the output of the stage.  The stores are non-temporal: the output is
+    // read again only by the next launch, long after it has left L2, and
+    // streaming it past L2 keeps the window rows there for their re-read by
+    // the stage update and the neighbouring tiles.  A short record is stored
+    // by its lane (put_out); a long one is put into the LDS staging (slot k: 0 for
+    // Out, 1 for A) and the whole wave then stores the row's contiguous span
+    // (flush), so every store instruction writes whole cache lines
+    auto put_out = [&](double val, gmem_t<double>* dst, size_t i, int k, int c) {
+      if constexpr (STAGE_ROWS) ost[(k * HYPE_TILE_W + lane - 1) * NV + c] = val;
+      else __builtin_nontemporal_store(val, &dst[i]);
+    };
+    auto flush = [&](gmem_t<double>* dst, int row, int k) {
+      wave_sync();
+      const size_t first = ((size_t)row * nx + x0) * NV;
+      const double* src = ost + k * HYPE_TILE_W * NV;
+      for (int e = lane; e < W * NV; e += HYPE_WAVE) __builtin_nontemporal_store(src[e], &dst[first + e]);
+      wave_sync();  // the staging is rewritten by the next row
+    };
+
+    // an operand of the stage update read from buffer B.  On the
+    // conservative path the window holds the stage input of the owned cells
+    // unchanged, so while the row is still in the window (wr) the input is
+    // taken from there instead of being read from memory a second time
+    auto operand = [&](gmem_t<const double>* B, size_t i, int row, int c, bool wr) -> double {
+      if constexpr (!PRIM) {
+        if (wr && B == Uin) return W_(slot(row), c, lane + 2);
+      }
+      return B[i];
+    };
+
     // assembly + stage update of cell (col, row) from the terms of its top
     // and bottom faces, its x-face part and its volume term
     // (rhs_function_prim.c:589-602)
-    auto assemble = [&](int row, const double* Gt, const double* Gb, const double* Et, const double* Eb,
-                        const double* xp, const double* nc) {
-      if (!owned || mode == HYPE_MODE_PRIME) return;
+    auto record = [&](int row, const double* Gt, const double* Gb, const double* Et, const double* Eb,
+                      const double* xp, const double* nc, bool wr) {
       const size_t idx = ((size_t)row * nx + col) * NV;
       double rhs[NV];
 #pragma unroll
@@ -470,23 +509,20 @@ struct Tile {
       }
       if (mode == HYPE_MODE_RHS) {
 #pragma unroll
-        for (int c = 0; c < NV; ++c) __builtin_nontemporal_store(rhs[c], &Out[idx + c]);
+        for (int c = 0; c < NV; ++c) put_out(rhs[c], Out, idx + c, 0, c);
       } else {
         // every operand of the record is loaded before the first store.  The
         // compiler cannot prove the output distinct from the inputs (it is
         // not, for the RK stages that update U in place), so loads
         // interleaved with the stores would each wait for memory: six round
-        // trips per row instead of one.  The stores are non-temporal: the
-        // output is read again only by the next launch, long after it has
-        // left L2, and streaming it past L2 keeps the window rows there for
-        // their re-read by this update and by the neighbouring tiles
+        // trips per row instead of one
         const auto out = Out;
         if (sd.op == 0) {
           double u[NV];
 #pragma unroll
-          for (int c = 0; c < NV; ++c) u[c] = Uin[idx + c];
+          for (int c = 0; c < NV; ++c) u[c] = operand(Uin, idx + c, row, c, wr);
 #pragma unroll
-          for (int c = 0; c < NV; ++c) __builtin_nontemporal_store(u[c] + cdt * rhs[c], &out[idx + c]);
+          for (int c = 0; c < NV; ++c) put_out(u[c] + cdt * rhs[c], out, idx + c, 0, c);
         } else if (sd.op == 2) {
           // RK104 after its fifth stage: w0 += dt/6 F (VecAXPY); w1 =
           // 1/25 U + 9/25 w0 (VecAXPBYPCZ, c = 0) into U; w0 = 15 w1 - 5 w0
@@ -495,15 +531,15 @@ struct Tile {
           double u[NV], o[NV];
 #pragma unroll
           for (int c = 0; c < NV; ++c) {
-            u[c] = Uin[idx + c];
+            u[c] = operand(Uin, idx + c, row, c, wr);
             o[c] = out[idx + c];
           }
 #pragma unroll
           for (int c = 0; c < NV; ++c) {
             const double w0 = u[c] + cdt * rhs[c];
             const double w1 = sd.a * o[c] + sd.g * w0;
-            __builtin_nontemporal_store(w1, &out[idx + c]);
-            __builtin_nontemporal_store(15.0 * w1 + -5.0 * w0, &A[idx + c]);
+            put_out(w1, out, idx + c, 0, c);
+            put_out(15.0 * w1 + -5.0 * w0, A, idx + c, 1, c);
           }
         } else {
           const auto X = Xb;
@@ -511,12 +547,25 @@ struct Tile {
           double x[NV], z[NV];
 #pragma unroll
           for (int c = 0; c < NV; ++c) {
-            x[c] = X[idx + c];
-            z[c] = Z[idx + c];
+            x[c] = operand(X, idx + c, row, c, wr);
+            z[c] = operand(Z, idx + c, row, c, wr);
           }
 #pragma unroll
-          for (int c = 0; c < NV; ++c) __builtin_nontemporal_store(sd.a * x[c] + cdt * rhs[c] + sd.g * z[c], &out[idx + c]);
+          for (int c = 0; c < NV; ++c) put_out(sd.a * x[c] + cdt * rhs[c] + sd.g * z[c], out, idx + c, 0, c);
         }
+      }
+    };
+    // called by the whole wave (the flush of staged rows needs every lane)
+    auto assemble = [&](int row, const double* Gt, const double* Gb, const double* Et, const double* Eb,
+                        const double* xp, const double* nc, bool wr) {
+      if constexpr (STAGE_ROWS) {
+        if (mode == HYPE_MODE_PRIME) return;
+        if (owned) record(row, Gt, Gb, Et, Eb, xp, nc, wr);
+        flush(Out, row, 0);
+        if (mode == HYPE_MODE_STAGE && sd.op == 2) flush(Ab, row, 1);
+      } else {
+        if (!owned || mode == HYPE_MODE_PRIME) return;
+        record(row, Gt, Gb, Et, Eb, xp, nc, wr);
       }
     };
 
@@ -714,7 +763,7 @@ struct Tile {
       }
 
       // ---- assembly + stage update of cell (col, r-1) --------------------
-      if (doAsm) assemble(r - 1, Gy, Gprev, Ey, Eprev, xpartPrev, ncpPrev);
+      if (doAsm) assemble(r - 1, Gy, Gprev, Ey, Eprev, xpartPrev, ncpPrev, true);  // rows r-2..r+2 are in the window
 
       // ---- carry -----------------------------------------------------------
 #pragma unroll
@@ -760,7 +809,7 @@ struct Tile {
         double Gm[NV], Em[NV];
         yface(lo, hi, alo, ahi, ym, Gm, Em);
         if (!upper) {
-          assemble(ym - 1, Gm, Gprev, Em, Eprev, xpartPrev, ncpPrev);
+          assemble(ym - 1, Gm, Gprev, Em, Eprev, xpartPrev, ncpPrev, false);
         } else {
           double Gt[NV], Et[NV], xp[NV], nc[NV];
 #pragma unroll
@@ -770,7 +819,7 @@ struct Tile {
             xp[c] = D_(2, c);
             nc[c] = D_(3, c);
           }
-          assemble(ym, Gt, Gm, Et, Em, xp, nc);
+          assemble(ym, Gt, Gm, Et, Em, xp, nc, false);  // row ym has left the window
         }
       }
     }
