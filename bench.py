@@ -413,6 +413,7 @@ def run_served(args):
         "latency_s": {"p50": round(float(np.percentile(tts, 50)), 4), "p99": round(float(np.percentile(tts, 99)), 4),
                       "makespan": round(makespan, 3)},
         "inadmissible_requests": int(sum(1 for _, _, _, inf, _ in res if inf.status.err)),
+        "inadmissible_ids": [j for j, (_, _, _, inf, _) in enumerate(res) if inf.status.err],
     }
     if args.no_solo:
         return out

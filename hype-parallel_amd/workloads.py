@@ -117,16 +117,18 @@ def dt_estimate(p, U):
 
 def served_draws(rng, count, rate):
     """Arrival time, system, side, steps and BN seed of every request of the
-    stream, without building the states."""
-    t = np.cumsum(rng.exponential(1.0 / rate, count))
+    stream, without building the states.  Each request's draws follow the
+    previous request's, so a shorter stream is a prefix of a longer one."""
+    t = 0.0
     out = []
     for k in range(count):
+        t += rng.exponential(1.0 / rate)
         n = int(np.exp(rng.uniform(np.log(64), np.log(2048))))
         steps = int(rng.integers(100, 2001))
         u = rng.uniform()
         system = H.EULER if u < 0.5 else (H.KAPILA if u < 0.85 else H.BAER_NUNZIATO)
         seed = int(rng.integers(1 << 62)) if system == H.BAER_NUNZIATO else 0
-        out.append((float(t[k]), system, n, steps, seed))
+        out.append((float(t), system, n, steps, seed))
     return out
 
 

@@ -11,6 +11,7 @@ import time
 import numpy as np
 import pytest
 
+import oracle_py as orc
 import workloads
 
 pytestmark = pytest.mark.gpu
@@ -58,3 +59,52 @@ def test_served_stream_prefix_equals_solo(hip, policy):
         assert inf.status.steps == st.steps and inf.status.err == st.err, f"request {j}: status differs"
         assert inf.status.t == st.t, f"request {j}: final time differs"
         assert np.array_equal(U, Us, equal_nan=True), f"request {j}: state differs from its solo run"
+
+
+# requests of the stream that end on an inadmissible state; 253 is the first,
+# a 65 x 65 Baer-Nunziato water-air interface whose gas pressure turns
+# negative at step 904 (8 of the first 1024 requests do, all Baer-Nunziato)
+INADMISSIBLE = 253
+NEIGHBOURS = (249, 250, 252, 254, 257)  # small requests that arrive around it
+
+
+def test_served_stream_inadmissible_request(hip):
+    """The stream's first inadmissible request, served among its small
+    neighbours, stops at the same step with the same state as its solo run;
+    the solo run stops where the oracle's solve loop does (step, offending
+    face and state), and the neighbours finish unaffected."""
+    draws = workloads.served_draws(np.random.default_rng(SEED), INADMISSIBLE + 5, RATE)
+    ks = sorted(NEIGHBOURS + (INADMISSIBLE,))
+    reqs = {k: workloads.served_request(*draws[k][1:]) for k in ks}
+    with hip.Server(policy=hip.SERVE_SHORTEST) as srv:
+        ids = {k: srv.submit(p, U0, t_end=t_end, max_steps=cap) for k, (p, U0, t_end, cap) in reqs.items()}
+        srv.drain()
+        res = {k: srv.result(i) for k, i in ids.items()}
+    for k in ks:
+        p, U0, t_end, cap = reqs[k]
+        Us, st = solo(hip, p, U0, t_end, cap)
+        U, inf = res[k]
+        print(f"[served-stream] request {k}: system {p.system} side {p.nx} steps {st.steps} err {st.err}",
+              flush=True)
+        assert inf.status.steps == st.steps and inf.status.err == st.err, f"request {k}: status differs"
+        assert np.array_equal(U, Us, equal_nan=True), f"request {k}: state differs from its solo run"
+        assert (st.err != 0) == (k == INADMISSIBLE), f"request {k}: unexpected status {st.err}"
+    p, U0, t_end, cap = reqs[INADMISSIBLE]
+    op = orc.problem(p.system, p.nx, p.ny, N=p.order, prim=bool(p.recons_prim),
+                     domain=(p.x_min, p.x_max, p.y_min, p.y_max), cfl=p.cfl, bc=tuple(p.bc), g1=p.gamma1,
+                     g2=p.gamma2, p1=p.pi1, p2=p.pi2, quirks=p.quirks)
+    ref = orc.solve(op, U0, cap, t_end=t_end, clip=True, threaded=True)
+    with hip.Context(p) as ctx:
+        ctx.set_state(U0)
+        ctx.begin(0.0, t_end, cap, hip.TS_CLIP | hip.TS_GRAPH)
+        ctx.advance(cap)
+        st = ctx.sync(allow_inadmissible=True)
+        msg, x, y = ctx.last_error()
+        U = ctx.get_state()
+    print(f"[served-stream] request {INADMISSIBLE}: {msg!r}; oracle steps {ref['steps']} kind {ref['err'].kind} "
+          f"at ({ref['err'].x}, {ref['err'].y})", flush=True)
+    assert ref["rc"] == hip.E_INADMISSIBLE and st.err == hip.E_INADMISSIBLE
+    assert st.steps == ref["steps"] == 904
+    assert (x, y) == (ref["err"].x, ref["err"].y)
+    assert np.array_equal(U, ref["U"], equal_nan=True)
+    assert np.array_equal(res[INADMISSIBLE][0], U, equal_nan=True)
