@@ -56,6 +56,10 @@ struct Euler {
   // from the shorter sweeps (256^2: 540 -> 601 Mcell-updates/s), large ones
   // from the second wave's latency hiding
   static constexpr int PAIR_MODES = 3;
+  // a split tile sends its CFL maximum to memory once, from the lower wave,
+  // instead of once per wave: the atomics of the last stage contend for one
+  // address (Euler quadrants 256^2: 636 -> 670 Mcell-updates/s)
+  static constexpr bool PAIR_CFL_COMBINE = true;
   static constexpr bool HAS_INLET = false;  // no InletBC in euler.c
   // no sound speed is shared between the admissibility check and the flux
   static constexpr bool HAS_ACACHE = false;
@@ -159,6 +163,10 @@ struct Kapila {
   // split tiles only: one reconstructed ring row per wave saved at the same
   // one wave per SIMD (1024 x 512 shock-bubble: 952 -> 985 Mcell-updates/s)
   static constexpr int PAIR_MODES = 2;
+  // not combined: the extra code after the row loop shifts the register
+  // allocation of the loop (Kapila 1024x512: -0.8 %, batched -0.5 %), more
+  // than the 493 fewer atomics gain
+  static constexpr bool PAIR_CFL_COMBINE = false;
   // left inlet of kapila_5eqn.c:289-320, 485-513 (tabulated per row by the host)
   static constexpr bool HAS_INLET = true;
   // the admissibility check at a node computes the a_sq the primitive flux
@@ -434,6 +442,7 @@ struct BN {
   // one wave per tile only: two windows of nine components would halve the
   // workgroups an LDS of 160 KB holds
   static constexpr int PAIR_MODES = 1;
+  static constexpr bool PAIR_CFL_COMBINE = false;  // one wave per tile
   static constexpr bool HAS_INLET = false;  // no InletBC in baer_nunziato.c
   // no sound speed is shared between the admissibility check and the flux
   static constexpr bool HAS_ACACHE = false;

@@ -210,7 +210,10 @@ struct Tile {
   // slower for 6
   static constexpr bool STAGE_ROWS = NV > 6;
   static constexpr int OST = STAGE_ROWS ? 2 * NV * HYPE_TILE_W : 0;  // Out and A
-  static constexpr int LDS = WAVES * WIN + (PAIR ? 2 * XCH + DEFER : 0) + WAVES * OST;
+  // and, where the pair combines its CFL maximum, one double: the upper
+  // wave's maximum
+  static constexpr bool COMBINE = PAIR && S::PAIR_CFL_COMBINE;
+  static constexpr int LDS = WAVES * WIN + (PAIR ? 2 * XCH + DEFER : 0) + WAVES * OST + (COMBINE ? 1 : 0);
 
   // value of component c at node/face point p from its modal coefficients
   // (rhs_function_prim.c:382-388, 425-440)
@@ -325,6 +328,7 @@ struct Tile {
     auto X_ = [&](int side, int q, int c) -> double& { return xch[side * XCH + (q * (NV + 1) + c) * HYPE_WAVE + lane]; };
     auto D_ = [&](int k, int c) -> double& { return defer[(k * NV + c) * HYPE_WAVE + lane]; };
     double* ost = lds + WAVES * WIN + (PAIR ? 2 * XCH + DEFER : 0) + (PAIR ? wave * OST : 0);
+    double* pair_smax = lds + WAVES * WIN + (PAIR ? 2 * XCH + DEFER : 0) + WAVES * OST;
     const int rbase = r_first - 2;  // window row of slot 0
     auto slot = [&](int r) { return (r - rbase) % NSLOT; };
     auto W_ = [&](int s, int c, int k) -> double& { return win[(s * NV + c) * COLS + k]; };
@@ -783,6 +787,8 @@ struct Tile {
       sync_window();  // the new row is visible to the next iteration
     }
 
+    const bool want_smax = (mode != HYPE_MODE_STAGE) || last_stage;
+
     // ---- the two halves of a split tile meet at face ym-1/2 -----------------
     if constexpr (PAIR) {
       if (split) {
@@ -794,7 +800,21 @@ struct Tile {
             X_(0, q, NV) = topA[q];
           }
         }
+        // the CFL maximum goes to memory once per workgroup: the upper wave
+        // hands its rows' maximum to the lower wave here.  After the barrier
+        // both waves solve the same middle face, and the upper wave adds no
+        // other face, so the lower wave's maximum then covers the tile
+        if constexpr (COMBINE) {
+          const double um = (want_smax && upper) ? wave_max(smax) : 0.0;
+          if (want_smax && upper && lane == 0) *pair_smax = um;
+        }
         __syncthreads();
+        if constexpr (COMBINE) {
+          if (want_smax && !upper) {
+            const double pm = *pair_smax;
+            if (pm > smax) smax = pm;
+          }
+        }
         double lo[NGF][NV], hi[NGF][NV], alo[NGF], ahi[NGF];
 #pragma unroll
         for (int q = 0; q < NGF; ++q) {
@@ -825,8 +845,7 @@ struct Tile {
     }
 
     // ---- CFL maximum and error word ----------------------------------------
-    const bool want_smax = (mode != HYPE_MODE_STAGE) || last_stage;
-    if (want_smax) {
+    if (want_smax && !(COMBINE && split && upper)) {
       const double m = wave_max(smax);
       if (lane == 0) {
         unsigned long long* slotp =
