@@ -472,9 +472,10 @@ struct Tile {
     // read again only by the next launch, long after it has left L2, and
     // streaming it past L2 keeps the window rows there for their re-read by
     // the stage update and the neighbouring tiles.  A short record is stored
-    // by its lane (put_out); a long one is put into the LDS staging (slot k: 0 for
-    // Out, 1 for A) and the whole wave then stores the row's contiguous span
-    // (flush), so every store instruction writes whole cache lines
+    // by its lane (put_out); a long one is put into the LDS staging (slot k:
+    // 0 for Out, 1 for A) and the whole wave then stores the row's
+    // contiguous span (flush), so every store instruction writes whole
+    // cache lines
     auto put_out = [&](double val, gmem_t<double>* dst, size_t i, int k, int c) {
       if constexpr (STAGE_ROWS) ost[(k * HYPE_TILE_W + lane - 1) * NV + c] = val;
       else __builtin_nontemporal_store(val, &dst[i]);

@@ -67,11 +67,12 @@ def test_steps_match_oracle(hip, system, prim, order):
     run_pair(hip, system, prim, order, (T, T, R, R), 80, 44, steps=12, seed=11, quirks=quirks)
 
 
-@pytest.mark.parametrize("system,order,layout", [
-    (0, 1, "one"), (0, 2, "one"), (0, 3, "one"), (0, 1, "split"), (0, 3, "split"),
-    (1, 2, "split"), (1, 3, "split"), (2, 1, "one"),
+@pytest.mark.parametrize("system,order,layout,bc", [
+    (0, 1, "one", "TTRR"), (0, 2, "one", "TTRR"), (0, 3, "one", "TTRR"), (0, 1, "split", "TTRR"),
+    (0, 3, "split", "TTRR"), (1, 2, "split", "TTRR"), (1, 3, "split", "TTRR"), (2, 1, "one", "TTRR"),
+    (0, 2, "split", "PPPP"), (1, 2, "split", "PPRR"), (2, 2, "one", "RRPP"), (2, 3, "one", "PPTT"),
 ])
-def test_steps_conservative_layouts(hip, system, order, layout):
+def test_steps_conservative_layouts(hip, system, order, layout, bc):
     """Conservative path, every stage-update form (AXPY, AXPBYPCZ with the
     stage input as X or Z, the RK104 fifth-stage update) in each tile layout
     the system is built for: the update takes the stage input of the owned
@@ -79,7 +80,7 @@ def test_steps_conservative_layouts(hip, system, order, layout):
     eos = states.EOS[system]
     nx, ny, steps = 76, 50, 6
     U0 = states.state(system, nx, ny, seed=21 + order)
-    bc = (T, T, R, R)
+    bc = tuple({"T": T, "R": R, "P": P}[k] for k in bc)
     dom = (0.0, 1.0, 0.0, ny / nx)
     p = hip.make_problem(system, nx, ny, order=order, prim=False, domain=dom, cfl=0.5, bc=bc, **eos)
     p.options |= hip.OPT_ONE_WAVE_TILES if layout == "one" else hip.OPT_SPLIT_TILES
@@ -88,7 +89,7 @@ def test_steps_conservative_layouts(hip, system, order, layout):
     ref = orc.solve(op, U0, steps)
     with hip.Context(p) as ctx:
         U, st = ctx.solve(U0, steps)
-    print(f"[steps-cons] sys={system} N={order} layout={layout} steps={st.steps}", flush=True)
+    print(f"[steps-cons] sys={system} N={order} layout={layout} bc={bc} steps={st.steps}", flush=True)
     assert st.steps == ref["steps"] == steps
     assert np.array_equal(U, ref["U"]), f"not bitwise equal: rel err {rel_err(U, ref['U']):.3e}"
 
