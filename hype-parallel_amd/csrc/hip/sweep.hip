@@ -213,7 +213,18 @@ struct Tile {
   // and, where the pair combines its CFL maximum, one double: the upper
   // wave's maximum
   static constexpr bool COMBINE = PAIR && S::PAIR_CFL_COMBINE;
-  static constexpr int LDS = WAVES * WIN + (PAIR ? 2 * XCH + DEFER : 0) + WAVES * OST + (COMBINE ? 1 : 0);
+  // and, on the primitive path of split tiles, per wave, the staging of the
+  // four window columns right of the lanes' 64 for 16 rows (they are
+  // converted 16 rows at a time).  Not in Euler's one-wave kernel: at two
+  // waves per SIMD it has no accumulation registers to spare and the
+  // staging spills it to scratch (served stream -2.5 %)
+  // The lower wave stages in its exchange area, which it writes only after
+  // its row loop; the upper wave needs its own (the upper half's exchange
+  // area and deferred state are filled in its first iterations)
+  static constexpr bool XBATCH = PRIM && PAIR;
+  static constexpr int XST = XBATCH ? 16 * 4 * NV : 0;
+  static_assert(XST <= XCH, "the lower wave's staging must fit its exchange area");
+  static constexpr int LDS = WAVES * WIN + (PAIR ? 2 * XCH + DEFER : 0) + WAVES * OST + (COMBINE ? 1 : 0) + XST;
 
   // value of component c at node/face point p from its modal coefficients
   // (rhs_function_prim.c:382-388, 425-440)
@@ -353,7 +364,7 @@ struct Tile {
       for (int c = 0; c < NV; ++c) q[c] = p[c];
       return (fx ? 1 : 0) | (fy ? 2 : 0);
     };
-    auto put = [&](int s, int k, double* q, int refl) {
+    auto conv = [&](double* q, int refl, double* v) {
       if (refl & 1) {
         q[S::REFX[0]] = -q[S::REFX[0]];
         if (S::REFX[1] >= 0) q[S::REFX[1]] = -q[S::REFX[1]];
@@ -362,15 +373,40 @@ struct Tile {
         q[S::REFY[0]] = -q[S::REFY[0]];
         if (S::REFY[1] >= 0) q[S::REFY[1]] = -q[S::REFY[1]];
       }
-      double v[NV];
       if (PRIM) {
         S::cons2prim(eos, q, v);
       } else {
 #pragma unroll
         for (int c = 0; c < NV; ++c) v[c] = q[c];
       }
+    };
+    auto put = [&](int s, int k, double* q, int refl) {
+      double v[NV];
+      conv(q, refl, v);
 #pragma unroll
       for (int c = 0; c < NV; ++c) W_(s, c, k) = v[c];
+    };
+    // the four window columns right of the lanes' 64 (grid columns x0+61 ..
+    // x0+64) on the primitive path: loaded and converted 16 rows at a time,
+    // one cell per lane, into the wave's staging xs ([c][row % 16][column]),
+    // and copied into the window a row at a time by lanes 0-3.  The
+    // conversion then runs once per 16 rows on every lane instead of every
+    // row on 4 lanes.  Window row j (counted from rbase) is staged by the
+    // batch of rows 16*(j/16) .. 16*(j/16)+15; rows past r_last+2 are not
+    // needed and not loaded
+    double* xs = wave == 0 ? xch : lds + WAVES * WIN + (PAIR ? 2 * XCH + DEFER : 0) + WAVES * OST + (COMBINE ? 1 : 0);
+    auto xrow = [&](int j0) { return rbase + j0 + (lane >> 2); };
+    auto xput = [&](double* q, int refl) {
+      double v[NV];
+      conv(q, refl, v);
+#pragma unroll
+      for (int c = 0; c < NV; ++c) xs[c * HYPE_WAVE + lane] = v[c];
+    };
+    auto xcopy = [&](int s, int j) {
+      if (lane < 4) {
+#pragma unroll
+        for (int c = 0; c < NV; ++c) W_(s, c, HYPE_WAVE + lane) = xs[c * HYPE_WAVE + (j & 15) * 4 + lane];
+      }
     };
     // left inlet (Kapila, rhs_function_prim.c:135-144): in the strip at the
     // left edge, the three window columns left of the grid take the state the
@@ -400,7 +436,23 @@ struct Tile {
 
     // prologue: rows r_first-2 .. r_first+2, all loads in flight before the
     // first store
-    {
+    if constexpr (XBATCH) {
+      double qa[5][NV], qx[NV];
+      int fa[5], fx = 0;
+      const bool xl = xrow(0) <= r_last + 2;
+#pragma unroll
+      for (int i = 0; i < 5; ++i) fa[i] = fetch(rbase + i, lane, qa[i]);
+      if (xl) fx = fetch(xrow(0), HYPE_WAVE + (lane & 3), qx);
+#pragma unroll
+      for (int i = 0; i < 5; ++i) {
+        put(slot(rbase + i), lane, qa[i], fa[i]);
+        put_inlet(slot(rbase + i), rbase + i);
+      }
+      if (xl) xput(qx, fx);
+      wave_sync();
+#pragma unroll
+      for (int i = 0; i < 5; ++i) xcopy(slot(rbase + i), i);
+    } else {
       double qa[5][NV], qb[5][NV];
       int fa[5], fb[5];
 #pragma unroll
@@ -577,11 +629,18 @@ struct Tile {
     for (int r = r_first; r <= r_last; ++r) {
       // prefetch the row that enters the window after this iteration
       const bool more = (r + 3 <= r_last + 2);
+      const int jn = r - r_first + 5;  // window row of row r+3, from rbase
+      const bool xb = XBATCH && more && (jn & 15) == 0;
+      const bool xl = xb && xrow(jn) <= r_last + 2;
       double qa[NV], qb[NV];
       int fa = 0, fb = 0;
       if (more) {
         fa = fetch(r + 3, lane, qa);
-        if (lane < 4) fb = fetch(r + 3, lane + HYPE_WAVE, qb);
+        if constexpr (XBATCH) {
+          if (xl) fb = fetch(xrow(jn), HYPE_WAVE + (lane & 3), qb);
+        } else {
+          if (lane < 4) fb = fetch(r + 3, lane + HYPE_WAVE, qb);
+        }
       }
 
       const bool ownedRow = (r >= own_lo) && (r < own_hi);
@@ -657,7 +716,15 @@ struct Tile {
       if (more) {
         const int sn = slot(r + 3);
         put(sn, lane, qa, fa);
-        if (lane < 4) put(sn, lane + HYPE_WAVE, qb, fb);
+        if constexpr (XBATCH) {
+          if (xb) {
+            if (xl) xput(qb, fb);
+            wave_sync();
+          }
+          xcopy(sn, jn);
+        } else {
+          if (lane < 4) put(sn, lane + HYPE_WAVE, qb, fb);
+        }
         put_inlet(sn, r + 3);
       }
 
