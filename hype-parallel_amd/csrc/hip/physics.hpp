@@ -163,10 +163,10 @@ struct Kapila {
   // split tiles only: one reconstructed ring row per wave saved at the same
   // one wave per SIMD (1024 x 512 shock-bubble: 952 -> 985 Mcell-updates/s)
   static constexpr int PAIR_MODES = 2;
-  // not combined: the extra code after the row loop shifts the register
-  // allocation of the loop (Kapila 1024x512: -0.8 %, batched -0.5 %), more
-  // than the 493 fewer atomics gain
-  static constexpr bool PAIR_CFL_COMBINE = false;
+  // one atomic per split tile (Kapila 1024x512: 1087 -> 1108
+  // Mcell-updates/s; the batched set, whose 64 instances already spread the
+  // atomics over 64 addresses, 1320 -> 1317)
+  static constexpr bool PAIR_CFL_COMBINE = true;
   // left inlet of kapila_5eqn.c:289-320, 485-513 (tabulated per row by the host)
   static constexpr bool HAS_INLET = true;
   // the admissibility check at a node computes the a_sq the primitive flux
