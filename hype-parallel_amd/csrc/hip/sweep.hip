@@ -217,7 +217,8 @@ struct Tile {
   // four window columns right of the lanes' 64 for 16 rows (they are
   // converted 16 rows at a time).  Not in Euler's one-wave kernel: at two
   // waves per SIMD it has no accumulation registers to spare and the
-  // staging spills it to scratch (served stream -2.5 %)
+  // staging spills it to scratch (served stream -2.5 %).
+  //
   // The lower wave stages in its exchange area, which it writes only after
   // its row loop; the upper wave needs its own (the upper half's exchange
   // area and deferred state are filled in its first iterations)
