@@ -151,3 +151,23 @@ def test_inlet_table_equals_the_oracle_profile(L, prim):
         assert np.array_equal(tab[j + 3], want), j
     p.bc = (H.BC_TRANSMISSIVE,) * 4
     assert L.hype_inlet_doubles(C.byref(p)) == 0
+
+
+def test_served_stream_prefixes_are_prefixes():
+    """Every request's draws follow the previous request's, so the 16-, 64-,
+    256- and 1,024-request traces the tests and benchmark replay are the
+    first requests of the one 2,400-request stream (20 requests/s for
+    120 s), with the stated mix and ranges."""
+    import workloads as WL
+    full = WL.served_draws(np.random.default_rng(2024), 2400, 20.0)
+    for n in (16, 64, 256, 1024):
+        assert WL.served_draws(np.random.default_rng(2024), n, 20.0) == full[:n]
+    t = np.array([d[0] for d in full])
+    assert np.all(np.diff(t) > 0) and 100.0 < t[-1] < 140.0
+    sides = np.array([d[2] for d in full])
+    steps = np.array([d[3] for d in full])
+    assert sides.min() >= 64 and sides.max() < 2048 and steps.min() >= 100 and steps.max() <= 2000
+    share = [np.mean([d[1] == s for d in full]) for s in (H.EULER, H.KAPILA, H.BAER_NUNZIATO)]
+    assert abs(share[0] - 0.50) < 0.04 and abs(share[1] - 0.35) < 0.04 and abs(share[2] - 0.15) < 0.03
+    # a BN request carries its own interface seed, the others none
+    assert all((d[4] != 0) == (d[1] == H.BAER_NUNZIATO) for d in full)
