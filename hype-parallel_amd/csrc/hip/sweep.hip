@@ -269,42 +269,10 @@ struct Tile {
     const bool leader = (T.flags & 1) != 0 && wave == 0;
     const int mode = L.mode;
 
-    // ---- time state (single writer per field per launch) -------------
     double dt = 0.0;
     int cur = 0;
     StageDesc sd = stage_desc(N, L.stage);
     const bool last_stage = (mode == HYPE_MODE_STAGE) && (L.stage == num_stages_dev(N) - 1);
-    if (mode == HYPE_MODE_STAGE) {
-      if (L.stage == 0) {
-        const int step = st->step;
-        const double t = st->t;
-        // stop after max_steps, at t_end, or after a step (or the priming
-        // call) that met an inadmissible state
-        if (step >= st->max_steps || !(t < st->t_end) || st->err[(step + 1) & 1] != ~0ull) {
-          if (leader && lane == 0) st->done = 1;
-          return;
-        }
-        const double sm = __longlong_as_double((long long)st->smax[step & 1]);
-        dt = (I->cfl * I->h) / (2.0 * sm);
-        if (st->clip && t + dt > st->t_end) dt = st->t_end - t;
-        cur = step;
-        if (leader && lane == 0) {
-          st->dt = dt;
-          st->cur = step;
-          st->smax[(step + 1) & 1] = 0ull;
-          if (st->dt_hist && step < st->hist_len) st->dt_hist[step] = dt;
-        }
-      } else {
-        if (st->done) return;
-        dt = st->dt;
-        cur = st->cur;
-      }
-      if (last_stage && leader && lane == 0) {
-        st->t = st->t + dt;
-        st->step = cur + 1;
-        st->err[(cur + 1) & 1] = ~0ull;  // the next step records there
-      }
-    }
     const auto Uin = gmem<const double>((mode == HYPE_MODE_STAGE) ? I->buf[sd.in] : (L.rhs_in ? L.rhs_in : I->buf[0]));
     // the stage update's buffers, read from the descriptor once per tile
     // rather than in every row iteration
@@ -312,7 +280,6 @@ struct Tile {
     const auto Xb = gmem<const double>(I->buf[sd.xb]);
     const auto Zb = gmem<const double>(I->buf[sd.zb]);
     const auto Ab = gmem(I->buf[1]);
-    const double cdt = dt / sd.bden;
 
     // ---- geometry ------------------------------------------------------
     const int x0 = T.x0, y0 = T.y0;
@@ -436,36 +403,73 @@ struct Tile {
     };
 
     // prologue: rows r_first-2 .. r_first+2, all loads in flight before the
-    // first store
-    if constexpr (XBATCH) {
-      double qa[5][NV], qx[NV];
-      int fa[5], fx = 0;
-      const bool xl = xrow(0) <= r_last + 2;
+    // first store, and before the time state is read: it decides whether the
+    // tile runs at all, and its round trip now overlaps theirs
+    double pqa[5][NV], pqb[5][NV];
+    int pfa[5], pfb[5];
+    const bool pxl = XBATCH && xrow(0) <= r_last + 2;
 #pragma unroll
-      for (int i = 0; i < 5; ++i) fa[i] = fetch(rbase + i, lane, qa[i]);
-      if (xl) fx = fetch(xrow(0), HYPE_WAVE + (lane & 3), qx);
+    for (int i = 0; i < 5; ++i) {
+      pfa[i] = fetch(rbase + i, lane, pqa[i]);
+      pfb[i] = 0;
+    }
+    if constexpr (XBATCH) {
+      if (pxl) pfb[0] = fetch(xrow(0), HYPE_WAVE + (lane & 3), pqb[0]);
+    } else {
+#pragma unroll
+      for (int i = 0; i < 5; ++i)
+        if (lane < 4) pfb[i] = fetch(rbase + i, lane + HYPE_WAVE, pqb[i]);
+    }
+
+    // ---- time state (single writer per field per launch) -------------
+    if (mode == HYPE_MODE_STAGE) {
+      if (L.stage == 0) {
+        const int step = st->step;
+        const double t = st->t;
+        // stop after max_steps, at t_end, or after a step (or the priming
+        // call) that met an inadmissible state
+        if (step >= st->max_steps || !(t < st->t_end) || st->err[(step + 1) & 1] != ~0ull) {
+          if (leader && lane == 0) st->done = 1;
+          return;
+        }
+        const double sm = __longlong_as_double((long long)st->smax[step & 1]);
+        dt = (I->cfl * I->h) / (2.0 * sm);
+        if (st->clip && t + dt > st->t_end) dt = st->t_end - t;
+        cur = step;
+        if (leader && lane == 0) {
+          st->dt = dt;
+          st->cur = step;
+          st->smax[(step + 1) & 1] = 0ull;
+          if (st->dt_hist && step < st->hist_len) st->dt_hist[step] = dt;
+        }
+      } else {
+        if (st->done) return;
+        dt = st->dt;
+        cur = st->cur;
+      }
+      if (last_stage && leader && lane == 0) {
+        st->t = st->t + dt;
+        st->step = cur + 1;
+        st->err[(cur + 1) & 1] = ~0ull;  // the next step records there
+      }
+    }
+    const double cdt = dt / sd.bden;
+
+    if constexpr (XBATCH) {
 #pragma unroll
       for (int i = 0; i < 5; ++i) {
-        put(slot(rbase + i), lane, qa[i], fa[i]);
+        put(slot(rbase + i), lane, pqa[i], pfa[i]);
         put_inlet(slot(rbase + i), rbase + i);
       }
-      if (xl) xput(qx, fx);
+      if (pxl) xput(pqb[0], pfb[0]);
       wave_sync();
 #pragma unroll
       for (int i = 0; i < 5; ++i) xcopy(slot(rbase + i), i);
     } else {
-      double qa[5][NV], qb[5][NV];
-      int fa[5], fb[5];
 #pragma unroll
       for (int i = 0; i < 5; ++i) {
-        fa[i] = fetch(rbase + i, lane, qa[i]);
-        fb[i] = 0;
-        if (lane < 4) fb[i] = fetch(rbase + i, lane + HYPE_WAVE, qb[i]);
-      }
-#pragma unroll
-      for (int i = 0; i < 5; ++i) {
-        put(slot(rbase + i), lane, qa[i], fa[i]);
-        if (lane < 4) put(slot(rbase + i), lane + HYPE_WAVE, qb[i], fb[i]);
+        put(slot(rbase + i), lane, pqa[i], pfa[i]);
+        if (lane < 4) put(slot(rbase + i), lane + HYPE_WAVE, pqb[i], pfb[i]);
         put_inlet(slot(rbase + i), rbase + i);
       }
     }
